@@ -1,0 +1,139 @@
+/*
+ * cc_gpu.h -- C-ABI of the MI355X (gfx950) implementation of the reference's
+ * finite-difference Jacobian assembly and Newton solve
+ * (tobjec/parallel-critical-collapse, src/NewtonSolver.cpp:615-757 and the
+ * shooting path below it).
+ *
+ * Plain C types only.  Every entry point returns a cc_status; no exception
+ * crosses the boundary.  Host arrays are caller-owned and every call is
+ * synchronous on the context's stream unless its name ends in _async.  One
+ * context per host thread.  Arrays of complex values are interleaved
+ * (re, im) doubles.
+ *
+ * Sizes: N = Ntau, M = N/2 complex modes per state, n = 3N/4 Newton unknowns.
+ * Newton vector layout (NewtonSolver.cpp:273-274, ICG.cpp:678-711):
+ *   x[0 : N/4)      Up   odd modes 1,3,..  as (re, im) pairs
+ *   x[N/4 : N/2)    psic odd modes
+ *   x[N/2 : 3N/4)   fc   even modes 0,2,.. as (re, im); x[N/2+1] holds the
+ *                   folded mode N/4 cosine and x[N/2+2] holds Delta.
+ */
+#ifndef CC_GPU_H
+#define CC_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cc_status {
+    CC_OK = 0,
+    CC_IRK_NOCONV = 1,      /* ShootingSolver.cpp:69-82, 112-125: IRK fixed point did not converge */
+    CC_DELTA_MISMATCH = 2,  /* ShootingSolver.cpp:167-178: left/right Delta differ */
+    CC_SINGULAR = 3,        /* NewtonSolver.cpp:751-755: dgesv info != 0 */
+    CC_BADARG = 4,
+    CC_DEVICE = 5           /* HIP runtime error; see cc_last_error */
+} cc_status;
+
+typedef enum cc_precision {
+    CC_FP64 = 0,            /* every shoot in FP64: the reference's arithmetic */
+    CC_MIXED = 1            /* perturbed shoots + LU in FP32, FP64 residual, FP64 refinement */
+} cc_precision;
+
+/* Problem description: the SimulationConfig keys the path reads
+ * (include/SimulationConfig.hpp:73-137) plus the precision policy. */
+typedef struct cc_problem {
+    int32_t Ntau;           /* 128, 256, 512, 1024 or 2048 */
+    int32_t SchemeIRK;      /* 2 (Gauss-Legendre 2 stages; the only scheme on the device) */
+    int32_t MaxIterIRK;
+    int32_t NLeft, NRight;
+    double  Dim;
+    double  XLeft, XMid, XRight;
+    double  PrecisionIRK;
+    double  EpsNewton;      /* absolute FD step (NewtonSolver.cpp:637, 644) */
+    /* precision policy (new keys; defaults reproduce the reference) */
+    int32_t precision;      /* cc_precision */
+    int32_t reuse_sides;    /* 1: a column that leaves one side's boundary data unchanged
+                               reuses the base shoot of that side (bit-identical result) */
+    double  PrecisionIRK32; /* IRK tolerance of FP32 shoots (0 = default 2e-6) */
+    double  EpsNewton32;    /* FD step of FP32 columns (0 = default 1e-3) */
+    int32_t refine_iters;   /* FP64 iterative-refinement sweeps of the Newton step (0 = default 3) */
+    int32_t reserved;
+} cc_problem;
+
+typedef struct cc_ctx cc_ctx;
+
+/* Context: owns device buffers, the stream, FFT twiddle tables and the
+ * x-grid (NewtonSolver.cpp:477-519).  Replaces the ICG/ShootingSolver/
+ * ODEStepper objects NewtonSolver.cpp:18-52 constructs. */
+cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out);
+cc_status cc_ctx_destroy(cc_ctx *ctx);
+const char *cc_last_error(const cc_ctx *ctx);
+/* Copies the context's x-grid (NLeft+NRight+1 doubles) to host memory. */
+cc_status cc_grid(cc_ctx *ctx, double *grid);
+
+/* F(x) in FP64: replaces NewtonSolver::shoot (NewtonSolver.cpp:403-423). */
+cc_status cc_residual(cc_ctx *ctx, const double *x, double *F);
+
+/* FD Jacobian: replaces NewtonSolver::assembleJacobian
+ * (NewtonSolver.cpp:615-702).  J is row-major n x n, J[j*n+i] = dF_j/dx_i.
+ * col_status (n int32, may be NULL) receives each column's cc_status.
+ * In CC_MIXED mode F0 is only used for shape checks: the difference quotient
+ * uses an FP32 base shoot computed in the same launch. */
+cc_status cc_jacobian(cc_ctx *ctx, const double *x, const double *F0, double *J, int32_t *col_status);
+
+/* Dense solve with partial pivoting: replaces LAPACKE_dgesv
+ * (NewtonSolver.cpp:739-757).  A (n x n row-major, 1 <= n <= 3 Ntau/4) is
+ * overwritten by its LU factors, b by the solution; *info follows LAPACK
+ * (0 ok, k > 0 zero pivot at k).  In CC_MIXED the factors are FP32 (returned
+ * widened) and the solution is refined in FP64. */
+cc_status cc_solve(cc_ctx *ctx, int32_t n, double *A, double *b, int32_t *info);
+
+/* One Newton step without host round trips: F0 = F(x), J, solve J dx = -F0
+ * (+ FP64 refinement in CC_MIXED).  Outputs F0 and dx (n doubles each);
+ * the update x += fac*dx stays with the caller (NewtonSolver.cpp:350-358). */
+cc_status cc_newton_step(cc_ctx *ctx, const double *x, double *F0, double *dx);
+
+/* Same as cc_newton_step on device-resident arrays (d_x, d_F0, d_dx are
+ * device pointers of n doubles); asynchronous on the context stream. */
+cc_status cc_newton_step_device_async(cc_ctx *ctx, const double *d_x, double *d_F0, double *d_dx);
+cc_status cc_synchronize(cc_ctx *ctx);
+
+/* Counters of the last Jacobian/Newton launch: [0] IRK steps, [1] fixed-point
+ * iterations (sum of itsReached), [2] chains integrated. */
+cc_status cc_counters(cc_ctx *ctx, int64_t out[3]);
+
+/* ---- unit entry points (the reference's lower-level interfaces) ---------- */
+
+/* SpectralTransformer forwardFFT / backwardFFT, complex-to-complex
+ * (SpectralTransformer.cpp:97-132) on `batch` signals of length Ntau.
+ * sign = +1: forward (exp(+i..), divided by N); sign = -1: backward.
+ * fp32 != 0 runs the FP32 transform (inputs/outputs still double). */
+cc_status cc_fft_batched(cc_ctx *ctx, const double *in, double *out, int32_t batch, int32_t sign,
+                         int32_t fp32);
+
+/* ODEStepper::computeDerivatives (ODEStepper.cpp:92-136) on `batch` states
+ * Y (M complex each) at abscissae xs (batch doubles); dY receives M complex
+ * each.  Delta of each state is Re Y[2] as in the reference. */
+cc_status cc_rhs_eval(cc_ctx *ctx, const double *Y, const double *xs, double *dY, int32_t batch,
+                      int32_t fp32);
+
+/* ShootingSolver::integrateToMatchPoint (ShootingSolver.cpp:53-145) over a
+ * caller-supplied grid for `batch` independent states: state b starts at
+ * grid index start[b] and steps to end[b].  its_total[b] receives the sum of
+ * itsReached; status[b] a cc_status.  With nsteps=1 this is
+ * ODEStepper::integrate (ODEStepper.cpp:144-150). */
+cc_status cc_integrate(cc_ctx *ctx, double *Y, const double *grid, int32_t ngrid,
+                       const int32_t *start, const int32_t *end, int32_t batch, int32_t fp32,
+                       int64_t *its_total, int32_t *status);
+
+/* InitialConditionGenerator boundary data for Newton vector x: unpack
+ * (ICG.cpp:719-758), computeLeftExpansion (37-239) and
+ * computeRightExpansion (252-667).  YL, YR: M complex each. */
+cc_status cc_expansions(cc_ctx *ctx, const double *x, double *YL, double *YR);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

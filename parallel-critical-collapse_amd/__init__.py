@@ -1,0 +1,226 @@
+"""ctypes front end of libcc_gpu.so (the C-ABI in include/cc_gpu.h).
+
+The library is the product: every method here calls a HIP entry point.  There
+is no CPU fallback; importing this package without the built library raises.
+
+    import importlib
+    cc = importlib.import_module("parallel-critical-collapse_amd")
+    ctx = cc.Context(cc.Problem.make(Ntau=128, NLeft=500, NRight=500, XMid=0.5))
+    F = ctx.residual(x)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libcc_gpu.so")
+HEADER = os.path.join(ROOT, "include", "cc_gpu.h")
+
+CC_OK, CC_IRK_NOCONV, CC_DELTA_MISMATCH, CC_SINGULAR, CC_BADARG, CC_DEVICE = range(6)
+CC_FP64, CC_MIXED = 0, 1
+STATUS_NAMES = {0: "OK", 1: "IRK_NOCONV", 2: "DELTA_MISMATCH", 3: "SINGULAR", 4: "BADARG", 5: "DEVICE"}
+
+
+class CCError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Problem(C.Structure):
+    """cc_problem: the SimulationConfig keys of the path plus the precision policy."""
+
+    _fields_ = [
+        ("Ntau", C.c_int32), ("SchemeIRK", C.c_int32), ("MaxIterIRK", C.c_int32),
+        ("NLeft", C.c_int32), ("NRight", C.c_int32),
+        ("Dim", C.c_double), ("XLeft", C.c_double), ("XMid", C.c_double), ("XRight", C.c_double),
+        ("PrecisionIRK", C.c_double), ("EpsNewton", C.c_double),
+        ("precision", C.c_int32), ("reuse_sides", C.c_int32),
+        ("PrecisionIRK32", C.c_double), ("EpsNewton32", C.c_double),
+        ("refine_iters", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+    @classmethod
+    def make(cls, Ntau, NLeft, NRight, Dim=4.0, XLeft=1e-3, XMid=0.5, XRight=0.999, SchemeIRK=2,
+             MaxIterIRK=100, PrecisionIRK=1e-15, EpsNewton=1e-8, precision=CC_FP64, reuse_sides=0,
+             PrecisionIRK32=0.0, EpsNewton32=0.0, refine_iters=0):
+        return cls(Ntau, SchemeIRK, MaxIterIRK, NLeft, NRight, Dim, XLeft, XMid, XRight, PrecisionIRK,
+                   EpsNewton, precision, reuse_sides, PrecisionIRK32, EpsNewton32, refine_iters, 0)
+
+    @classmethod
+    def from_shape(cls, shape, **kw):
+        return cls.make(Ntau=shape.Ntau, NLeft=shape.NLeft, NRight=shape.NRight, Dim=shape.Dim,
+                        XLeft=shape.XLeft, XMid=shape.XMid, XRight=shape.XRight,
+                        SchemeIRK=shape.SchemeIRK, MaxIterIRK=shape.MaxIterIRK,
+                        PrecisionIRK=shape.PrecisionIRK, EpsNewton=shape.EpsNewton, **kw)
+
+
+def declared_symbols():
+    """Entry points declared in include/cc_gpu.h."""
+    with open(HEADER) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"^\s*(?:cc_status|const char \*)\s*(cc_\w+)\s*\(", text, re.M)))
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+_vp = C.c_void_p
+_lib = None
+
+
+def load():
+    """Load libcc_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python {HERE}/build.py` "
+                          "(or __graft_entry__.build()) first")
+    L = C.CDLL(LIB_PATH)
+    L.cc_ctx_create.argtypes = [C.POINTER(Problem), C.POINTER(_vp)]
+    L.cc_ctx_destroy.argtypes = [_vp]
+    L.cc_last_error.argtypes = [_vp]
+    L.cc_last_error.restype = C.c_char_p
+    L.cc_grid.argtypes = [_vp, _dp]
+    L.cc_counters.argtypes = [_vp, _lp]
+    L.cc_residual.argtypes = [_vp, _dp, _dp]
+    L.cc_jacobian.argtypes = [_vp, _dp, _dp, _dp, _ip]
+    L.cc_solve.argtypes = [_vp, C.c_int32, _dp, _dp, _ip]
+    L.cc_newton_step.argtypes = [_vp, _dp, _dp, _dp]
+    L.cc_newton_step_device_async.argtypes = [_vp, _vp, _vp, _vp]
+    L.cc_synchronize.argtypes = [_vp]
+    L.cc_fft_batched.argtypes = [_vp, _dp, _dp, C.c_int32, C.c_int32, C.c_int32]
+    L.cc_rhs_eval.argtypes = [_vp, _dp, _dp, _dp, C.c_int32, C.c_int32]
+    L.cc_integrate.argtypes = [_vp, _dp, _dp, C.c_int32, _ip, _ip, C.c_int32, C.c_int32, _lp, _ip]
+    L.cc_expansions.argtypes = [_vp, _dp, _dp, _dp]
+    for name in declared_symbols():
+        getattr(L, name).restype = C.c_char_p if name == "cc_last_error" else C.c_int
+    _lib = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _arr(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Context:
+    """A cc_ctx: device buffers, streams and tables for one problem."""
+
+    def __init__(self, problem: Problem):
+        L = load()
+        self.problem = problem
+        h = _vp()
+        st = L.cc_ctx_create(C.byref(problem), C.byref(h))
+        if st != CC_OK:
+            raise CCError(st, "cc_ctx_create failed")
+        self.h = h
+        self.N = problem.Ntau
+        self.M = self.N // 2
+        self.n = 3 * self.N // 4
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().cc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != CC_OK:
+            raise CCError(st, f"{what}: {load().cc_last_error(self.h).decode()}")
+
+    def grid(self):
+        g = np.zeros(self.problem.NLeft + self.problem.NRight + 1)
+        self._check(load().cc_grid(self.h, _d(g)), "cc_grid")
+        return g
+
+    def counters(self):
+        out = np.zeros(3, np.int64)
+        load().cc_counters(self.h, out.ctypes.data_as(_lp))
+        return out
+
+    def residual(self, x):
+        x = _arr(x)
+        F = np.zeros(self.n)
+        self._check(load().cc_residual(self.h, _d(x), _d(F)), "cc_residual")
+        return F
+
+    def jacobian(self, x, F0, check=True):
+        x, F0 = _arr(x), _arr(F0)
+        J = np.zeros((self.n, self.n))
+        cs = np.zeros(self.n, np.int32)
+        st = load().cc_jacobian(self.h, _d(x), _d(F0), _d(J), cs.ctypes.data_as(_ip))
+        if check:
+            self._check(st, "cc_jacobian")
+        return J, cs, st
+
+    def solve(self, A, b):
+        A = np.array(A, np.float64, order="C")
+        b = np.array(b, np.float64)
+        info = C.c_int32(0)
+        st = load().cc_solve(self.h, A.shape[0], _d(A), _d(b), C.byref(info))
+        return b, info.value, A, st
+
+    def newton_step(self, x):
+        x = _arr(x)
+        F0, dx = np.zeros(self.n), np.zeros(self.n)
+        self._check(load().cc_newton_step(self.h, _d(x), _d(F0), _d(dx)), "cc_newton_step")
+        return F0, dx
+
+    def newton_step_device(self, d_x: int, d_F0: int, d_dx: int):
+        """Device pointers (e.g. torch tensor data_ptr()); asynchronous."""
+        self._check(load().cc_newton_step_device_async(self.h, _vp(d_x), _vp(d_F0), _vp(d_dx)),
+                    "cc_newton_step_device_async")
+
+    def synchronize(self):
+        self._check(load().cc_synchronize(self.h), "cc_synchronize")
+
+    def fft(self, X, sign, fp32=False):
+        X = _arr(np.atleast_2d(X), np.complex128)
+        out = np.zeros_like(X)
+        self._check(load().cc_fft_batched(self.h, X.ctypes.data_as(_dp), out.ctypes.data_as(_dp),
+                                          X.shape[0], sign, int(fp32)), "cc_fft_batched")
+        return out
+
+    def rhs(self, Y, xs, fp32=False):
+        Y = _arr(np.atleast_2d(Y), np.complex128)
+        xs = _arr(np.atleast_1d(xs))
+        dY = np.zeros_like(Y)
+        self._check(load().cc_rhs_eval(self.h, Y.ctypes.data_as(_dp), _d(xs), dY.ctypes.data_as(_dp),
+                                       Y.shape[0], int(fp32)), "cc_rhs_eval")
+        return dY
+
+    def integrate(self, Y, grid, start, end, fp32=False):
+        Y = np.array(np.atleast_2d(Y), np.complex128)
+        grid = _arr(grid)
+        start = _arr(np.atleast_1d(start), np.int32)
+        end = _arr(np.atleast_1d(end), np.int32)
+        its = np.zeros(Y.shape[0], np.int64)
+        status = np.zeros(Y.shape[0], np.int32)
+        self._check(load().cc_integrate(self.h, Y.ctypes.data_as(_dp), _d(grid), grid.size,
+                                        start.ctypes.data_as(_ip), end.ctypes.data_as(_ip), Y.shape[0],
+                                        int(fp32), its.ctypes.data_as(_lp), status.ctypes.data_as(_ip)),
+                    "cc_integrate")
+        return Y, its, status
+
+    def expansions(self, x):
+        x = _arr(x)
+        YL = np.zeros(self.M, np.complex128)
+        YR = np.zeros(self.M, np.complex128)
+        self._check(load().cc_expansions(self.h, _d(x), YL.ctypes.data_as(_dp), YR.ctypes.data_as(_dp)),
+                    "cc_expansions")
+        return YL, YR

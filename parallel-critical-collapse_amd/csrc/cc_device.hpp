@@ -1,0 +1,213 @@
+// cc_device.hpp -- device building blocks for gfx950: complex arithmetic,
+// in-register DFTs and the register-resident Stockham FFT with LDS exchange.
+//
+// Layout convention used by every kernel ("layout L"): a workgroup of T
+// threads owns a length-N signal; thread t holds elements t + T*p for
+// p = 0 .. P-1 (P = N/T) in registers.  Loads and stores of layout L are
+// unit-stride across a wavefront.
+//
+// Transform conventions are those of the reference's SpectralTransformer
+// (SpectralTransformer.cpp:33-38): SIGN = +1 computes
+// X_k = sum_j x_j exp(+2 pi i jk/N) (the reference's "forwardFFT", which then
+// divides by N), SIGN = -1 computes exp(-2 pi i jk/N) unscaled ("backwardFFT").
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ccd {
+
+template <typename R>
+struct cx {
+    R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R>
+__device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R>
+__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ cx<R> cscale(cx<R> a, R s) { return {a.x * s, a.y * s}; }
+
+// a * (SIGN * i)
+template <int SIGN, typename R>
+__device__ __forceinline__ cx<R> mul_si(cx<R> a)
+{
+    if constexpr (SIGN > 0) return {-a.y, a.x};
+    else return {a.y, -a.x};
+}
+
+// cos/sin(2 pi k / 16), k = 0..15, for the in-register DFTs (R <= 16)
+__device__ constexpr double kC16[16] = {
+    1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508977,
+    0.0, -0.38268343236508977, -0.70710678118654752, -0.92387953251128674,
+    -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508977,
+    0.0, 0.38268343236508977, 0.70710678118654752, 0.92387953251128674};
+__device__ constexpr double kS16[16] = {
+    0.0, 0.38268343236508977, 0.70710678118654752, 0.92387953251128674,
+    1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508977,
+    0.0, -0.38268343236508977, -0.70710678118654752, -0.92387953251128674,
+    -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508977};
+
+// w^K * a with w = exp(SIGN 2 pi i / RAD), K a compile-time constant
+template <int RAD, int K, int SIGN, typename R>
+__device__ __forceinline__ cx<R> twiddle_const(cx<R> a)
+{
+    constexpr int KK = K % RAD;
+    if constexpr (KK == 0) {
+        return a;
+    } else if constexpr (4 * KK == RAD) {
+        return mul_si<SIGN>(a);
+    } else if constexpr (2 * KK == RAD) {
+        return {-a.x, -a.y};
+    } else if constexpr (4 * KK == 3 * RAD) {
+        return mul_si<-SIGN>(a);
+    } else {
+        constexpr int idx = KK * (16 / RAD);
+        const R c = (R)kC16[idx];
+        const R s = (R)(SIGN * kS16[idx]);
+        return {a.x * c - a.y * s, a.x * s + a.y * c};
+    }
+}
+
+template <int RAD, int K, int SIGN, typename R>
+__device__ __forceinline__ void combine(cx<R>* v, const cx<R>* e, const cx<R>* o)
+{
+    if constexpr (K < RAD / 2) {
+        const cx<R> t = twiddle_const<RAD, K, SIGN>(o[K]);
+        v[K] = e[K] + t;
+        v[K + RAD / 2] = e[K] - t;
+        combine<RAD, K + 1, SIGN>(v, e, o);
+    }
+}
+
+// In-register DFT of length RAD (power of two <= 16), natural order in/out.
+template <int RAD, int SIGN, typename R>
+__device__ __forceinline__ void dft(cx<R>* v)
+{
+    if constexpr (RAD == 1) {
+        return;
+    } else if constexpr (RAD == 2) {
+        const cx<R> a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    } else {
+        cx<R> e[RAD / 2], o[RAD / 2];
+#pragma unroll
+        for (int i = 0; i < RAD / 2; ++i) {
+            e[i] = v[2 * i];
+            o[i] = v[2 * i + 1];
+        }
+        dft<RAD / 2, SIGN>(e);
+        dft<RAD / 2, SIGN>(o);
+        combine<RAD, 0, SIGN>(v, e, o);
+    }
+}
+
+// One Stockham pass (Govindaraju et al., SC'08 formulation) of a batch of B
+// signals held in layout L; recurses to the next pass.  NS is the length of
+// the sub-transforms already done.  The radix is P while N/NS >= P, then the
+// remaining factor.  All but the last pass exchange through LDS (B*N
+// complex); the last pass's outputs land back in layout L.
+// tw[e] = (cos, sin)(2 pi e / N), e in [0, N).
+template <int N, int T, int SIGN, int B, int NS, typename R>
+__device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const cx<R>* __restrict__ tw,
+                                         int t)
+{
+    constexpr int P = N / T;
+    constexpr int RAD = (N / NS >= P) ? P : N / NS;
+    constexpr bool LAST = (NS * RAD == N);
+    constexpr int NB = P / RAD;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = t + T * b;
+        const int k = j & (NS - 1);
+        cx<R> w[RAD];
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < RAD; ++r) {
+                cx<R> tr = tw[r * k * (N / (NS * RAD))];
+                if constexpr (SIGN < 0) tr.y = -tr.y;
+                w[r] = tr;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < B; ++s) {
+            cx<R> u[RAD];
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) u[r] = v[s][b + r * NB];
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < RAD; ++r) u[r] = cmul(u[r], w[r]);
+            }
+            dft<RAD, SIGN>(u);
+            if constexpr (LAST) {
+#pragma unroll
+                for (int r = 0; r < RAD; ++r) v[s][b + r * NB] = u[r];
+            } else {
+                const int base = (j / NS) * NS * RAD + k;
+#pragma unroll
+                for (int r = 0; r < RAD; ++r) lds[s * N + base + r * NS] = u[r];
+            }
+        }
+    }
+    if constexpr (!LAST) {
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < B; ++s)
+#pragma unroll
+            for (int p = 0; p < P; ++p) v[s][p] = lds[s * N + t + T * p];
+        __syncthreads();
+        fft_pass<N, T, SIGN, B, NS * RAD>(v, lds, tw, t);
+    }
+}
+
+// Batched block FFT: B signals of length N in layout L, in place.
+// Requires N/T >= 2 and a __syncthreads-safe caller (all T threads call it).
+template <int N, int T, int SIGN, int B, typename R>
+__device__ __forceinline__ void fft_block(cx<R> (&v)[B][N / T], cx<R>* lds, const cx<R>* __restrict__ tw, int t)
+{
+    static_assert((N & (N - 1)) == 0 && N / T >= 2 && N / T <= 16, "unsupported FFT shape");
+    fft_pass<N, T, SIGN, B, 1>(v, lds, tw, t);
+}
+
+// Single-signal convenience wrapper.
+template <int N, int T, int SIGN, typename R>
+__device__ __forceinline__ void fft1(cx<R> (&v)[N / T], cx<R>* lds, const cx<R>* __restrict__ tw, int t)
+{
+    cx<R>(&vb)[1][N / T] = *reinterpret_cast<cx<R>(*)[1][N / T]>(&v);
+    fft_block<N, T, SIGN, 1>(vb, lds, tw, t);
+}
+
+// Workgroup sum of one value per thread (T threads, T a multiple of 64).
+// red: LDS scratch of at least T/64 values.  Returns the sum to every thread.
+template <int T, typename A>
+__device__ __forceinline__ A block_sum(A v, A* red, int t)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if constexpr (T == 64) {
+        return v;
+    } else {
+        if ((t & 63) == 0) red[t >> 6] = v;
+        __syncthreads();
+        A s = red[0];
+#pragma unroll
+        for (int w = 1; w < T / 64; ++w) s += red[w];
+        __syncthreads();
+        return s;
+    }
+}
+
+// Spectral wavenumber m of index k in a length-N spectrum (SpectralTransformer.cpp:150)
+template <int N>
+__device__ __forceinline__ int wavenumber(int k)
+{
+    return (k < N / 2) ? k : k - N;
+}
+
+}  // namespace ccd

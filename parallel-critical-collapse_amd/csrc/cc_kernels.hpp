@@ -1,0 +1,138 @@
+// cc_kernels.hpp -- per-column kernels around the chain kernel and the unit
+// transform kernel.  All of them use one workgroup per column (or signal).
+#pragma once
+#include "cc_chain.hpp"
+#include "cc_lu.hpp"
+#include "cc_spectral.hpp"
+
+namespace ccd {
+
+// Boundary data of one side for a list of columns (NewtonSolver.cpp:403-416
+// then ICG.cpp:37-239 or 252-667).  Column b perturbs slot cols[b] of x by
+// eps (cols[b] < 0: the unperturbed base), writes its state to
+// Y[chain[b]] in the chain precision and Delta to delta[chain[b]].
+template <int N, int T, typename R, bool LEFT>
+__global__ __launch_bounds__(T) void k_boundary(const double *x, const int32_t *cols, const int32_t *chain,
+                                                 double eps, double Xb, double Dim, const cx<double> *tw,
+                                                 cx<R> *Y, double *delta)
+{
+    constexpr int P = N / T, H = P / 2, M = N / 2;
+    __shared__ cx<double> xb[N];
+    __shared__ double bc[4];
+    const int t = threadIdx.x;
+    const int b = blockIdx.x;
+    const int col = cols[b];
+    const int dslot = N / 2 + 2;
+    const double Delta = (col == dslot) ? x[dslot] + eps : x[dslot];
+    Spec<N, T> S{t, xb, bc, tw, 2.0 * 3.14159265358979323846 / Delta};
+    double Up[P], psic[P], fc[P];
+    unpack_fields<N, T>(S, x, col, eps, Up, psic, fc);
+    cx<double> Yh[H];
+    if constexpr (LEFT)
+        left_expansion<N, T>(S, Xb, Dim, fc, psic, Yh);
+    else
+        right_expansion<N, T>(S, Xb, Dim, Up, Yh);
+    if (t == 2) Yh[0].x = Delta;                    // ICG.cpp:195 / 624
+    const int ch = chain[b];
+#pragma unroll
+    for (int h = 0; h < H; ++h) Y[(size_t)ch * M + t + T * h] = {(R)Yh[h].x, (R)Yh[h].y};
+    if (t == 0) delta[ch] = Delta;
+}
+
+// Mismatch at the match point and its packing (ShootingSolver.cpp:154-182,
+// NewtonSolver.cpp:420-422).  Column b reads chain states lidx[b], ridx[b]
+// and writes F[b*n ...].
+template <int N, int T, typename R>
+__global__ __launch_bounds__(T) void k_mismatch(const cx<R> *Y, const int32_t *lidx, const int32_t *ridx,
+                                                 const double *delta, double precision, const cx<double> *tw,
+                                                 double *F, int32_t *status)
+{
+    constexpr int P = N / T, H = P / 2, M = N / 2, n = 3 * N / 4;
+    __shared__ cx<double> xb[N];
+    __shared__ double bc[4];
+    const int t = threadIdx.x;
+    const int b = blockIdx.x;
+    const int li = lidx[b], ri = ridx[b];
+    // FP32 chains do not carry Delta at full precision: take it from the chain record
+    const double DL = delta[li], DR = delta[ri];
+    Spec<N, T> S{t, xb, bc, tw, 2.0 * 3.14159265358979323846 / DL};
+    cx<double> mis[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        const int q = t + T * h;
+        const cx<R> l = Y[(size_t)li * M + q], r = Y[(size_t)ri * M + q];
+        cx<double> L = {(double)l.x, (double)l.y}, Rr = {(double)r.x, (double)r.y};
+        if (q == 2) {
+            L.x = DL;
+            Rr.x = DR;
+        }
+        mis[h] = Rr - L;
+        if (q == 2) {
+            if (mis[h].x > precision) status[b] = 2;   // ShootingSolver.cpp:167
+            mis[h].x = L.x;                             // ShootingSolver.cpp:181
+        }
+    }
+    state_to_packed<N, T>(S, mis, F + (size_t)b * n);
+}
+
+// J[j*n + c] = (F[c][j] - F[base][j]) / eps  (NewtonSolver.cpp:682-685)
+__global__ void k_fd(const double *F, const int32_t *cols, int ncols, int base, int n, double eps, double *J)
+{
+    const int j = blockIdx.y;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ncols || j >= n) return;
+    const int c = cols[b];
+    if (c < 0) return;
+    J[(size_t)j * n + c] = (F[(size_t)b * n + j] - F[(size_t)base * n + j]) / eps;
+}
+
+// Batched length-N transform with the reference's scaling (unit entry point).
+template <int N, int T, typename R, int SIGN>
+__global__ __launch_bounds__(T) void k_fft(const cx<double> *in, cx<double> *out, const cx<R> *tw)
+{
+    constexpr int P = N / T;
+    __shared__ cx<R> xb[N];
+    const int t = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * N;
+    cx<R> v[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const cx<double> a = in[base + t + T * p];
+        v[p] = {(R)a.x, (R)a.y};
+    }
+    fft1<N, T, SIGN>(v, xb, tw, t);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        cx<R> r = v[p];
+        if constexpr (SIGN > 0) r = cscale(r, (R)1 / (R)N);
+        out[base + t + T * p] = {(double)r.x, (double)r.y};
+    }
+}
+
+template <typename R>
+__global__ void k_cast_to(const double *in, R *out, size_t count)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = (R)in[i];
+}
+
+template <typename R>
+__global__ void k_cast_from(const R *in, double *out, size_t count)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = (double)in[i];
+}
+
+__global__ void k_negate_copy(const double *in, double *out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = -in[i];
+}
+
+__global__ void k_axpy(double *x, const double *d, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += d[i];
+}
+
+}  // namespace ccd

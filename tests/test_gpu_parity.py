@@ -1,0 +1,221 @@
+"""Parity of the HIP path (through the C-ABI) against the oracle.
+
+Tolerances (FP64 path vs the FP64 oracle; both compute the same arithmetic
+with different FFT rounding):
+  transforms, series, RHS:      1e-13 normwise relative
+  one IRK step / a whole side:  1e-11 normwise relative
+  residual F(x):                1e-9 relative to max|Y| at the match point
+  Jacobian (FD, h = 1e-8):      1e-5 Frobenius relative
+  Newton step dx:               1e-4 normwise relative
+FP32 unit transforms: 2e-6 normwise relative.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from _helpers import cc, ctx_for, golden, oracle_for, shape_c1
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
+
+
+@pytest.mark.parametrize("N", [128, 256, 512, 1024, 2048])
+def test_fft_matches_dft(N):
+    """Transform conventions of SpectralTransformer.cpp:33-38 against numpy's DFT."""
+    ctx = cc.Context(cc.Problem.make(Ntau=N, NLeft=10, NRight=10))
+    rng = np.random.default_rng(N)
+    X = rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))
+    fwd = ctx.fft(X, +1)
+    bwd = ctx.fft(X, -1)
+    assert rel(fwd, np.fft.ifft(X, axis=1)) < 1e-14
+    assert rel(bwd, np.fft.fft(X, axis=1)) < 1e-14
+    assert rel(ctx.fft(X, +1, fp32=True), np.fft.ifft(X, axis=1)) < 2e-6
+    # round trip backward(forward(x)) = x (test/test_fft.cpp:89-90)
+    assert rel(ctx.fft(fwd, -1), X) < 1e-14
+
+
+def test_expansions_match_golden():
+    """Boundary states at Ntau=128 against test/test_init_gen.cpp:323-371."""
+    g = golden("init_gen_128.json")
+    o = O.Oracle(O.Config(Ntau=128, Delta=g["Delta"]))
+    x = o.pack(g["Up"], g["psic"], g["fc"])
+    x[128 // 2 + 2] = g["Delta"]
+    ctx = cc.Context(cc.Problem.make(Ntau=128, NLeft=10, NRight=10, XLeft=g["XLeft"], XRight=g["XRight"]))
+    YL, YR = ctx.expansions(x)
+    rl = np.array(g["Yleft_ref"]) @ [1, 1j]
+    rr = np.array(g["Yright_ref"]) @ [1, 1j]
+    assert np.max(np.abs(YL - rl)) < 2e-14
+    assert np.max(np.abs(YR - rr)) < 2e-14
+
+
+@pytest.mark.parametrize("N", [128, 512, 1024])
+def test_expansions_match_oracle(N):
+    s = shape_c1(N)
+    x = s["x"]
+    o = oracle_for(s)
+    F, st, (YL0, YR0, _, _) = o.shoot_states(x) if N == 128 else (None, 0, (None,) * 4)
+    if N != 128:
+        up, ps, fc = o.unpack(np.where(np.arange(x.size) == N // 2 + 2, 0.0, x))
+        YL0 = o.left_expansion(fc, ps, x[N // 2 + 2])
+        YR0 = o.right_expansion(up, x[N // 2 + 2])
+    YL, YR = ctx_for(s).expansions(x)
+    assert rel(YL, YL0) < 1e-13
+    assert rel(YR, YR0) < 1e-13
+
+
+@pytest.mark.parametrize("N", [128, 1024])
+def test_rhs_matches_oracle(N):
+    """ODEStepper::computeDerivatives on boundary states and a mid-grid state."""
+    s = shape_c1(N)
+    x = s["x"]
+    o = oracle_for(s)
+    up, ps, fc = o.unpack(np.where(np.arange(x.size) == N // 2 + 2, 0.0, x))
+    YL = o.left_expansion(fc, ps, x[N // 2 + 2])
+    YR = o.right_expansion(up, x[N // 2 + 2])
+    Ys = np.stack([YL, YR])
+    xs = np.array([s["cfg"].XLeft * 1.0007, s["cfg"].XRight * 0.9993])
+    ref = np.stack([o.derivatives(Ys[i], xs[i]) for i in range(2)])
+    ctx = ctx_for(s)
+    got = ctx.rhs(Ys, xs)
+    for i in range(2):
+        assert rel(got[i], ref[i]) < 1e-12, i
+    got32 = ctx.rhs(Ys, xs, fp32=True)
+    for i in range(2):
+        assert rel(got32[i], ref[i]) < 1e-3, i
+
+
+def test_irk_steps_match_oracle():
+    """ODEStepper::stepIRK: state and iteration counts over the first 20 steps."""
+    s = shape_c1(128)
+    o = oracle_for(s)
+    x = s["x"]
+    up, ps, fc = o.unpack(np.where(np.arange(x.size) == 128 // 2 + 2, 0.0, x))
+    YL = o.left_expansion(fc, ps, x[128 // 2 + 2])
+    grid = o.grid()
+    Y = YL.copy()
+    its_ref = 0
+    for i in range(20):
+        Y, conv, its = o.step_irk(Y, grid[i], grid[i + 1])
+        assert conv
+        its_ref += its
+    ctx = ctx_for(s)
+    got, its, st = ctx.integrate(YL[None, :], grid, [0], [20])
+    assert st[0] == 0
+    assert rel(got[0], Y) < 1e-12
+    assert abs(int(its[0]) - its_ref) <= 2
+
+
+def test_sides_match_oracle_c1():
+    """A whole left and right integration at the 1000-point shape."""
+    s = shape_c1(128)
+    o = oracle_for(s)
+    F, st, (YL0, YR0, YLm, YRm) = o.shoot_states(s["x"])
+    assert st == 0
+    ctx = ctx_for(s)
+    iM, iR = s["cfg"].NLeft, s["cfg"].NLeft + s["cfg"].NRight
+    got, its, stat = ctx.integrate(np.stack([YL0, YR0]), o.grid(), [0, iR], [iM, iM])
+    assert list(stat) == [0, 0]
+    assert rel(got[0], YLm) < 1e-11
+    assert rel(got[1], YRm) < 1e-11
+
+
+def test_residual_matches_oracle_c1():
+    s = shape_c1(128)
+    o = oracle_for(s)
+    F_ref, st, (_, _, YLm, _) = o.shoot_states(s["x"])
+    assert st == 0
+    F = cc_residual(s)
+    scale = np.max(np.abs(YLm))
+    assert np.max(np.abs(F - F_ref)) / scale < 1e-9
+
+
+def cc_residual(s):
+    return ctx_for(s).residual(s["x"])
+
+
+@pytest.fixture(scope="module")
+def c1_reference():
+    """Oracle Jacobian and Newton step at the 1000-point shape (8 threads, ~20 s)."""
+    s = shape_c1(128)
+    o = oracle_for(s)
+    F0, st = o.shoot(s["x"])
+    assert st == 0
+    J, status, cnt, worst = O.jacobian(s["cfg"], s["x"], F0)
+    assert worst == 0
+    dx, info, _, _ = O.dgesv(J, -F0)
+    assert info == 0
+    return s, F0, J, dx, cnt
+
+
+def test_jacobian_matches_oracle_c1(c1_reference):
+    s, F0, J_ref, dx_ref, cnt = c1_reference
+    ctx = ctx_for(s)
+    J, cs, st = ctx.jacobian(s["x"], F0)
+    assert st == 0 and not cs.any()
+    assert rel(J, J_ref) < 1e-5
+    # counters: same IRK work as the oracle within a few iterations per column
+    c = ctx.counters()
+    assert c[0] == cnt[0]
+    assert abs(c[1] - cnt[1]) / cnt[1] < 0.02
+
+
+def test_jacobian_reuse_sides_is_identical(c1_reference):
+    s, F0, _, _, _ = c1_reference
+    J_full, _, _ = ctx_for(s).jacobian(s["x"], F0)
+    J_reuse, _, _ = ctx_for(s, reuse_sides=1).jacobian(s["x"], F0)
+    assert np.array_equal(J_full, J_reuse)
+
+
+def test_newton_step_matches_oracle_c1(c1_reference):
+    s, F0_ref, J_ref, dx_ref, _ = c1_reference
+    F0, dx = ctx_for(s).newton_step(s["x"])
+    assert np.max(np.abs(F0 - F0_ref)) < 1e-9
+    assert rel(dx, dx_ref) < 1e-4
+
+
+def test_solve_golden_10x10():
+    """test/test_lapack.cpp:26-74: solution 1..10 within 1e-12."""
+    g = golden("lapack_10.json")
+    ctx = cc.Context(cc.Problem.make(Ntau=128, NLeft=10, NRight=10))
+    x, info, _, st = ctx.solve(np.array(g["A"]), np.array(g["b"]))
+    assert st == 0 and info == 0
+    assert np.max(np.abs(x - np.array(g["x"]))) < 1e-12
+
+
+@pytest.mark.parametrize("n", [7, 64, 96, 200, 768])
+def test_solve_matches_oracle_dgesv(n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + np.eye(n) * 0.1
+    b = rng.standard_normal(n)
+    ref, info_ref, LU_ref, _ = O.dgesv(A, b)
+    ctx = cc.Context(cc.Problem.make(Ntau=1024, NLeft=10, NRight=10))
+    x, info, LU, st = ctx.solve(A, b)
+    assert st == 0 and info == 0 and info_ref == 0
+    assert rel(x, ref) < 1e-10
+    assert rel(LU, LU_ref) < 1e-10       # same pivot sequence -> same factors
+
+
+def test_solve_singular_reports_info():
+    A = np.ones((5, 5))
+    ctx = cc.Context(cc.Problem.make(Ntau=128, NLeft=10, NRight=10))
+    _, info, _, st = ctx.solve(A, np.ones(5))
+    _, info_ref, _, _ = O.dgesv(A, np.ones(5))
+    assert st == cc.CC_SINGULAR and info == info_ref == 2
+
+
+def test_mixed_solve_refines_to_fp64():
+    rng = np.random.default_rng(5)
+    n = 384
+    A = rng.standard_normal((n, n)) + np.eye(n) * 2
+    b = rng.standard_normal(n)
+    ref = np.linalg.solve(A, b)
+    ctx = cc.Context(cc.Problem.make(Ntau=512, NLeft=10, NRight=10, precision=cc.CC_MIXED))
+    x, info, _, st = ctx.solve(A, b)
+    assert st == 0 and info == 0
+    assert rel(x, ref) < 1e-12
