@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Newton-step throughput of the HIP path on one MI355X.
+
+One step = F(x) (FP64) + the n finite-difference Jacobian columns + the dense
+solve of J dx = -F: the unit NewtonSolver::run times as NewtonStep_i
+(NewtonSolver.cpp:281-365).  Workload: Ntau=1024, 8000 x-points (NLeft =
+NRight = 4000), XMid=0.5, Dim=4, seeded anchored input (workload.py), inputs
+resident in HBM before the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  value = Newton steps/s of the mixed-precision policy
+(FP32 columns and LU, FP64 residual and refinement); the FP64 form is timed
+in the same run and reported beside it.
+"""
+import argparse
+import importlib
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def flops_chain(N, its, steps):
+    """Algorithmic FLOPs of the chain kernels: per fixed-point iteration
+    2 F_RHS + 18 N (two RHS evaluations and the stage update), per IRK step
+    6 N (final combination); F_RHS = 25 N log2 N + 100 N."""
+    frhs = 25 * N * math.log2(N) + 100 * N
+    return its * (2 * frhs + 18 * N) + steps * 6 * N
+
+
+def time_policy(cc, dev, shape, precision, steps, warmup, reuse):
+    n = shape.n
+    ctx = cc.Context(cc.Problem.from_shape(shape, precision=precision, reuse_sides=int(reuse)))
+    x = importlib.import_module("parallel-critical-collapse_amd.workload").initial_x(shape)
+    d_x, d_F0, d_dx = dev.DeviceArray(x), dev.DeviceArray(np.zeros(n)), dev.DeviceArray(np.zeros(n))
+    ctx.profile(True)
+    name = "mixed" if precision == cc.CC_MIXED else "fp64"
+    for w in range(warmup):
+        t = time.perf_counter()
+        ctx.newton_step_device(d_x.ptr, d_F0.ptr, d_dx.ptr)
+        ctx.synchronize()
+        log(f"[bench] {name} warmup {w + 1}/{warmup}: {time.perf_counter() - t:.3f} s")
+    dev.synchronize()
+    chain_ms, launches, its, nsteps = 0.0, 0, 0, 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.newton_step_device(d_x.ptr, d_F0.ptr, d_dx.ptr)
+        ctx.synchronize()
+        ms, nl, it, st = ctx.chain_timing()
+        chain_ms += ms
+        launches += nl
+        its += it
+        nsteps += st
+        log(f"[bench] {name} step {k + 1}/{steps}: chain kernels {ms / 1e3:.3f} s, its {it}")
+    dev.synchronize()
+    wall = time.perf_counter() - t0
+    F0, dx = d_F0.download(), d_dx.download()
+    ctx.close()
+    return {
+        "wall_s": wall, "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
+        "chain_ms": chain_ms, "launches": launches, "its": its, "irk_steps": nsteps,
+        "kbar": its / max(nsteps, 1), "F0_rms": float(np.sqrt(np.mean(F0 ** 2))), "dx": dx,
+    }
+
+
+def cpu_baseline(shape, threads, budget_s):
+    """The oracle (CPU restatement, OpenMP over Jacobian columns like the
+    reference's cc_openmp build) on a bounded sample: one column per thread,
+    spread over the Up, psic, fc slots and Delta; scaled to Newton steps/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    n = shape.n
+    cfg = O.Config(Ntau=shape.Ntau, NLeft=shape.NLeft, NRight=shape.NRight, XMid=shape.XMid,
+                   XLeft=shape.XLeft, XRight=shape.XRight, Dim=shape.Dim, Delta=shape.Delta,
+                   EpsNewton=shape.EpsNewton, PrecisionIRK=shape.PrecisionIRK, MaxIterIRK=shape.MaxIterIRK)
+    W = importlib.import_module("parallel-critical-collapse_amd.workload")
+    x = W.initial_x(shape)
+    cols = sorted(set(int(round(v)) for v in np.linspace(0, n - 1, threads)) | {shape.Ntau // 2 + 2})[:threads]
+    F0 = np.zeros(n)
+    t = time.perf_counter()
+    J, status, cnt, worst = O.jacobian(cfg, x, F0, cols=cols, nthreads=threads)
+    dt = time.perf_counter() - t
+    cols_per_s = len(cols) / dt
+    return {
+        "value": cols_per_s / (n + 1), "unit": "Newton steps/s", "cores": threads, "kind": "port",
+        "sample": f"{len(cols)} FD Jacobian columns (full shoots, Ntau={shape.Ntau}, {shape.X} x-points) "
+                  f"in {dt:.1f} s on {threads} threads; steps/s = columns/s / (n+1), n={n}",
+        "status_ok": bool(worst == 0), "kbar": float(cnt[1] / max(cnt[0], 1)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=3, help="workload shape 1, 2, 3 or 5")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 policy")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the column shards of a multi-GPU run are described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    cc = importlib.import_module("parallel-critical-collapse_amd")
+    dev = importlib.import_module("parallel-critical-collapse_amd.device")
+    W = importlib.import_module("parallel-critical-collapse_amd.workload")
+    shape = W.config(args.config)
+    log(f"[bench] workload Ntau={shape.Ntau} X={shape.X} n={shape.n}")
+
+    mixed = time_policy(cc, dev, shape, cc.CC_MIXED, args.steps, args.warmup, False)
+    fp64 = None if args.no_fp64 else time_policy(cc, dev, shape, cc.CC_FP64, args.steps, args.warmup, False)
+
+    N = shape.Ntau
+    fl = flops_chain(N, mixed["its"], mixed["irk_steps"])
+    achieved = fl / (mixed["chain_ms"] / 1e3) / 1e12
+    peak = 157.3
+    roof = {"bound": "mfma", "pipe": "fp32 valu (FFT + pointwise; no dense contraction)",
+            "kernel": "k_chain<1024,256,float>", "achieved": round(achieved, 3), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+            "flops_per_step": fl, "chain_kernel_s_per_step": mixed["chain_ms"] / 1e3 / args.steps,
+            "avg_launch_ms": mixed["chain_ms"] / max(mixed["launches"], 1)}
+    out = {
+        "metric": "Jacobian+Newton steps/s at Ntau=1024, 8000 x-pts (mixed FP32/FP64)",
+        "value": round(mixed["steps_per_s"], 5), "unit": "Newton steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(mixed["ms_per_step"], 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic: shipped D=4 solution + seeded N(0,1)*0.1/k^2 odd/even-harmonic perturbation",
+        "config": {"workload": f"newton_step Ntau={shape.Ntau} x_points={shape.X} n={shape.n} Dim={shape.Dim} "
+                               f"XMid={shape.XMid} IRK2", "policy": "mixed"},
+        "kbar_mixed": round(mixed["kbar"], 3),
+        "roofline": roof,
+    }
+    if fp64 is not None:
+        fl64 = flops_chain(N, fp64["its"], fp64["irk_steps"])
+        out["fp64"] = {"value": round(fp64["steps_per_s"], 5), "ms_per_step": round(fp64["ms_per_step"], 2),
+                       "kbar": round(fp64["kbar"], 3),
+                       "achieved_tflops": round(fl64 / (fp64["chain_ms"] / 1e3) / 1e12, 3), "peak": 78.6}
+        out["fp64"]["frac"] = round(out["fp64"]["achieved_tflops"] / 78.6, 4)
+        out["mixed_over_fp64"] = round(mixed["steps_per_s"] / fp64["steps_per_s"], 3)
+        d64 = fp64["dx"]
+        out["dx_rel_err_mixed_vs_fp64"] = float(np.linalg.norm(mixed["dx"] - d64) / np.linalg.norm(d64))
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline on {args.cpu_threads} threads")
+        out["cpu_baseline"] = cpu_baseline(shape, args.cpu_threads, 30.0)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -104,6 +104,14 @@ cc_status cc_synchronize(cc_ctx *ctx);
  * iterations (sum of itsReached), [2] chains integrated. */
 cc_status cc_counters(cc_ctx *ctx, int64_t out[3]);
 
+/* Chain-kernel timing (the reference's AssembleStep_i timer,
+ * NewtonSolver.cpp:326-335, at kernel granularity).  When enabled, HIP events
+ * bracket the IRK chain kernels of every launch on the main stream;
+ * cc_chain_timing reports the last launch set: elapsed ms, number of kernel
+ * launches, fixed-point iterations and IRK steps they executed. */
+cc_status cc_profile(cc_ctx *ctx, int32_t enable);
+cc_status cc_chain_timing(cc_ctx *ctx, double *ms, int32_t *launches, int64_t *its, int64_t *steps);
+
 /* ---- unit entry points (the reference's lower-level interfaces) ---------- */
 
 /* SpectralTransformer forwardFFT / backwardFFT, complex-to-complex

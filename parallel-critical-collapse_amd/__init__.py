@@ -99,6 +99,8 @@ def load():
     L.cc_rhs_eval.argtypes = [_vp, _dp, _dp, _dp, C.c_int32, C.c_int32]
     L.cc_integrate.argtypes = [_vp, _dp, _dp, C.c_int32, _ip, _ip, C.c_int32, C.c_int32, _lp, _ip]
     L.cc_expansions.argtypes = [_vp, _dp, _dp, _dp]
+    L.cc_profile.argtypes = [_vp, C.c_int32]
+    L.cc_chain_timing.argtypes = [_vp, _dp, _ip, _lp, _lp]
     for name in declared_symbols():
         getattr(L, name).restype = C.c_char_p if name == "cc_last_error" else C.c_int
     _lib = L
@@ -185,6 +187,17 @@ class Context:
         """Device pointers (e.g. torch tensor data_ptr()); asynchronous."""
         self._check(load().cc_newton_step_device_async(self.h, _vp(d_x), _vp(d_F0), _vp(d_dx)),
                     "cc_newton_step_device_async")
+
+    def profile(self, enable=True):
+        self._check(load().cc_profile(self.h, int(enable)), "cc_profile")
+
+    def chain_timing(self):
+        """(ms, launches, fixed-point iterations, IRK steps) of the last profiled launch set."""
+        ms, nl = C.c_double(0), C.c_int32(0)
+        its, steps = C.c_int64(0), C.c_int64(0)
+        self._check(load().cc_chain_timing(self.h, C.byref(ms), C.byref(nl), C.byref(its), C.byref(steps)),
+                    "cc_chain_timing")
+        return ms.value, nl.value, its.value, steps.value
 
     def synchronize(self):
         self._check(load().cc_synchronize(self.h), "cc_synchronize")

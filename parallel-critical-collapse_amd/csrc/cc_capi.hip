@@ -52,6 +52,8 @@ struct cc_ctx {
     double *grid = nullptr;
     cx<double> *tw64 = nullptr;
     cx<float> *tw32 = nullptr;
+    double *invm64 = nullptr;            // 1/m per spectral index (0 at DC and Nyquist)
+    float *invm32 = nullptr;
     double *x = nullptr, *F0 = nullptr, *dx = nullptr, *r = nullptr, *d = nullptr;
     double *J = nullptr;                 // n x n
     double *A = nullptr;                 // n x n LU work (FP64)
@@ -61,6 +63,12 @@ struct cc_ctx {
     ColumnBuffers base;                  // the concurrent FP64 base shoot of the mixed policy
     int maxcols = 0, maxchains = 0;
     std::vector<void *> allocs;
+    // chain-kernel timing of the main stream
+    bool profile = false;
+    hipEvent_t tk0 = nullptr, tk1 = nullptr;
+    int32_t prof_launches = 0;
+    int64_t prof_its = 0, prof_steps = 0;
+    bool prof_pending = false;
 };
 
 namespace {
@@ -210,10 +218,13 @@ cc_status launch_columns(cc_ctx *c, const Plan &P, const ColumnBuffers &B, const
     a.status = B.status;
     a.fail_idx = B.fail;
     a.its = B.its;
-    if constexpr (sizeof(R) == 8)
+    if constexpr (sizeof(R) == 8) {
         a.tw = reinterpret_cast<const cx<R> *>(c->tw64);
-    else
+        a.invm = reinterpret_cast<const R *>(c->invm64);
+    } else {
         a.tw = reinterpret_cast<const cx<R> *>(c->tw32);
+        a.invm = reinterpret_cast<const R *>(c->invm32);
+    }
     a.maxIts = c->p.MaxIterIRK;
     a.Dim = c->p.Dim;
     a.precision = precision;
@@ -226,10 +237,19 @@ cc_status launch_columns(cc_ctx *c, const Plan &P, const ColumnBuffers &B, const
     a.c0 = c->c[0];
     a.c1 = c->c[1];
     const int maxsteps = *std::max_element(P.nsteps.begin(), P.nsteps.end());
+    const bool timed = c->profile && s == c->s0;
+    if (timed) HIPCHK(c, hipEventRecord(c->tk0, s));
+    int launches = 0;
     for (int s0 = 0; s0 < maxsteps; s0 += kStepsPerLaunch) {
         a.step_begin = s0;
         a.step_end = std::min(maxsteps, s0 + kStepsPerLaunch);
         hipLaunchKernelGGL((ccd::k_chain<N, T, R>), dim3(nch), dim3(T), 0, s, a);
+        ++launches;
+    }
+    if (timed) {
+        HIPCHK(c, hipEventRecord(c->tk1, s));
+        c->prof_launches = launches;
+        c->prof_pending = true;
     }
     HIPCHK(c, hipGetLastError());
     hipLaunchKernelGGL((ccd::k_mismatch<N, T, R>), dim3(ncols), dim3(T), 0, s, Y, B.lidx, B.ridx, B.delta,
@@ -289,6 +309,10 @@ cc_status collect_status(cc_ctx *c, const Plan &P, const ColumnBuffers &B, hipSt
         c->counters[0] += steps;
         c->counters[1] += itsum;
         c->counters[2] += P.nchains;
+        if (s == c->s0) {
+            c->prof_its = itsum;
+            c->prof_steps = steps;
+        }
     }
     cc_status worst = CC_OK;
     for (size_t b = 0; b < P.cols.size(); ++b) {
@@ -454,6 +478,8 @@ cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
     if (bad(hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking), "stream")) return CC_DEVICE;
     if (bad(hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking), "stream")) return CC_DEVICE;
     if (bad(hipEventCreateWithFlags(&c->ev, hipEventDisableTiming), "event")) return CC_DEVICE;
+    if (bad(hipEventCreate(&c->tk0), "event")) return CC_DEVICE;
+    if (bad(hipEventCreate(&c->tk1), "event")) return CC_DEVICE;
     const int n = c->n, M = c->M;
     c->maxcols = n + 2;
     c->maxchains = 2 * c->maxcols;
@@ -463,6 +489,8 @@ cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
     ALLOC(c->grid, c->X + 1);
     ALLOC(c->tw64, N);
     ALLOC(c->tw32, N);
+    ALLOC(c->invm64, N);
+    ALLOC(c->invm32, N);
     ALLOC(c->x, n);
     ALLOC(c->F0, n);
     ALLOC(c->dx, n);
@@ -511,6 +539,15 @@ cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
         }
         if (bad(hipMemcpy(c->tw64, t64.data(), sizeof(cx<double>) * N, hipMemcpyHostToDevice), "tw")) return CC_DEVICE;
         if (bad(hipMemcpy(c->tw32, t32.data(), sizeof(cx<float>) * N, hipMemcpyHostToDevice), "tw")) return CC_DEVICE;
+        std::vector<double> im64(N);
+        std::vector<float> im32(N);
+        for (int k = 0; k < N; ++k) {
+            const int m = k < N / 2 ? k : k - N;
+            im64[k] = (k == 0 || k == N / 2) ? 0.0 : 1.0 / (double)m;
+            im32[k] = (float)im64[k];
+        }
+        if (bad(hipMemcpy(c->invm64, im64.data(), sizeof(double) * N, hipMemcpyHostToDevice), "invm")) return CC_DEVICE;
+        if (bad(hipMemcpy(c->invm32, im32.data(), sizeof(float) * N, hipMemcpyHostToDevice), "invm")) return CC_DEVICE;
     }
     *out = c;
     return CC_OK;
@@ -523,6 +560,8 @@ cc_status cc_ctx_destroy(cc_ctx *c)
     (void)hipStreamSynchronize(c->s1);
     for (void *q : c->allocs) (void)hipFree(q);
     (void)hipEventDestroy(c->ev);
+    (void)hipEventDestroy(c->tk0);
+    (void)hipEventDestroy(c->tk1);
     (void)hipStreamDestroy(c->s0);
     (void)hipStreamDestroy(c->s1);
     delete c;
@@ -542,6 +581,31 @@ cc_status cc_counters(cc_ctx *c, int64_t out[3])
 {
     if (!c || !out) return CC_BADARG;
     for (int i = 0; i < 3; ++i) out[i] = c->counters[i];
+    return CC_OK;
+}
+
+cc_status cc_profile(cc_ctx *c, int32_t enable)
+{
+    if (!c) return CC_BADARG;
+    c->profile = enable != 0;
+    c->prof_pending = false;
+    return CC_OK;
+}
+
+cc_status cc_chain_timing(cc_ctx *c, double *ms, int32_t *launches, int64_t *its, int64_t *steps)
+{
+    if (!c || !ms || !launches || !its || !steps) return CC_BADARG;
+    if (!c->prof_pending) {
+        c->err = "no profiled chain launch since cc_profile(ctx, 1)";
+        return CC_BADARG;
+    }
+    HIPCHK(c, hipEventSynchronize(c->tk1));
+    float f = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&f, c->tk0, c->tk1));
+    *ms = f;
+    *launches = c->prof_launches;
+    *its = c->prof_its;
+    *steps = c->prof_steps;
     return CC_OK;
 }
 
@@ -702,12 +766,12 @@ cc_status cc_rhs_eval(cc_ctx *c, const double *Y, const double *xs, double *dY, 
         if (fp32) {
             hipLaunchKernelGGL((ccd::k_cast_to<float>), dim3(g), dim3(256), 0, c->s0, dtmp, (float *)dYin, 2 * nY);
             hipLaunchKernelGGL((ccd::k_rhs_pairs<NN, T, float>), dim3(batch), dim3(T), 0, c->s0,
-                               (const cx<float> *)dYin, dx_, dd, (cx<float> *)dYout, c->tw32, c->p.Dim);
+                               (const cx<float> *)dYin, dx_, dd, (cx<float> *)dYout, c->tw32, c->invm32, c->p.Dim);
             hipLaunchKernelGGL((ccd::k_cast_from<float>), dim3(g), dim3(256), 0, c->s0, (const float *)dYout, dtmp,
                                2 * nY);
         } else {
             hipLaunchKernelGGL((ccd::k_rhs_pairs<NN, T, double>), dim3(batch), dim3(T), 0, c->s0,
-                               (const cx<double> *)dtmp, dx_, dd, (cx<double> *)dYout, c->tw64, c->p.Dim);
+                               (const cx<double> *)dtmp, dx_, dd, (cx<double> *)dYout, c->tw64, c->invm64, c->p.Dim);
             (void)hipMemcpyAsync(dtmp, dYout, nY * sizeof(cx<double>), hipMemcpyDeviceToDevice, c->s0);
         }
     });
@@ -776,6 +840,7 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
             a.fail_idx = dfail;
             a.its = dits;
             a.tw = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? (const void *)c->tw64 : (const void *)c->tw32);
+            a.invm = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c->invm64 : (const void *)c->invm32);
             a.maxIts = c->p.MaxIterIRK;
             a.Dim = c->p.Dim;
             a.precision = sizeof(R) == 8 ? c->p.PrecisionIRK : c->p.PrecisionIRK32;

@@ -1,11 +1,11 @@
 // cc_chain.hpp -- the hot path: IRK2 integration of many shooting chains.
 //
 // One workgroup integrates one chain (one side of one Jacobian column) from
-// its boundary to the match point, entirely on chip: the state y, the two
-// stage iterates and the two stage derivatives live in registers (state
-// slots: thread t owns modes t + T*h, h < P/2), every FFT is the register/LDS
-// Stockham transform of cc_device.hpp.  Each workgroup runs its own
-// fixed-point loop, so every chain stops iterating exactly when the
+// its boundary to the match point, entirely on chip: the step-start state,
+// the two stage iterates and the candidate next state live in registers
+// (state slots: thread t owns modes t + T*h, h < P/2), every FFT is the
+// register/LDS Stockham transform of cc_device.hpp.  Each workgroup runs its
+// own fixed-point loop, so every chain stops iterating exactly when the
 // reference would (ODEStepper.cpp:271-275) and chains never wait on each
 // other.
 //
@@ -17,11 +17,20 @@
 //   SpectralTransformer::solveInhom SpectralTransformer.cpp:264-292
 //   ShootingSolver::integrateToMatchPoint ShootingSolver.cpp:53-145
 //
-// The two stages' RHS evaluations are independent within one fixed-point
-// iteration, so they are evaluated together: the real-input transforms of
-// solveInhom for stage 0 and stage 1 are packed into one complex transform
-// (a + i b), which makes 10 length-N transforms per iteration instead of the
-// reference's 18 (9 per RHS in its serial/OpenMP build).
+// Work organisation of one fixed-point iteration (both stages together):
+//  * per stage: doubled spectrum + tau-derivative, one batched inverse FFT of
+//    2 signals, half-period split; the RHS is rewritten as
+//        dU/dx = A_U - B_U r,  dV/dx = A_V - B_V r,  df/dx = C_F (r - 1),
+//    r = 1/IA2 = a^2 (ODEStepper.cpp:111-121 regrouped), so only A_U, B_U,
+//    A_V, B_V, C_F and Coeff1 survive the split;
+//  * solveInhom for both stages at once: stage 0 and stage 1 are packed into
+//    one complex transform (a + i b) for each of its four transforms; the
+//    lambda = 0 integration commutes with the packing, the lambda = f0
+//    integration is applied after separating the two Hermitian spectra;
+//    r = e^{F} / X comes from the e^{F} already needed for h = -g e^{F}
+//    (SpectralTransformer.cpp:276-291), so no e^{-F} is formed;
+//  * per stage: FieldsToStateVector as one batched forward FFT of 2 signals.
+// 10 length-N transforms per iteration instead of the reference's 18.
 #pragma once
 #include "cc_device.hpp"
 
@@ -39,6 +48,7 @@ struct ChainParams {
     int32_t *fail_idx;        // grid index of the failing step
     int64_t *its;             // sum of itsReached over the chain
     const cx<R> *tw;          // twiddles e^{2 pi i e/N}
+    const R *invm;            // 1/m at spectral index k (0 at k = 0 and N/2)
     int32_t step_begin, step_end;  // chunk of each chain's step sequence
     int32_t maxIts;
     double Dim;
@@ -48,26 +58,29 @@ struct ChainParams {
 
 template <typename R>
 struct RhsConst {
-    R dm2, dm3, c3, two_dm3, mg, q2;   // D-2, D-3, (D-2)^3, 2(D-3), -(D-3), (k0 N/2)^2
+    R k0, inv_k0;                     // 2 pi / Delta and its inverse
+    R dm2, dm3, c3, two_dm3, mg, q2;  // D-2, D-3, (D-2)^3, 2(D-3), -(D-3), (k0 N/2)^2
 };
 
 // Evaluate f_s = RHS(yk_s, xs_s) for both stages (ODEStepper.cpp:92-136).
-// wk[p]  = m k0 at the thread's spectral slot p (0 at Nyquist)
-// imk[p] = 1/(m k0), 0 at m = 0 and at Nyquist
+// bc[0], bc[1]: Re of mode M-2 of each stage state (the mode-2 fix value),
+// published by the caller before the call.
 template <int N, int T, typename R>
 __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> (&f)[2][N / T / 2],
-                                         const R (&xs)[2], const RhsConst<R> &K, const R (&wk)[N / T],
-                                         const R (&imk)[N / T], cx<R> *xb, R *bc, const cx<R> *__restrict__ tw,
-                                         int t)
+                                         const R (&xs)[2], const RhsConst<R> &K, const R *__restrict__ invm,
+                                         cx<R> *xb, R *bc, const cx<R> *__restrict__ tw, int t)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
     const R half = (R)0.5;
     const R invN = (R)1 / (R)N;
 
-    // (1) doubled spectrum + mode-2 fix (ICG.cpp:797-800), tau-derivative (803)
-    cx<R> z[2][2][P];   // [stage][field | derivative][p]
+    // per-stage products of the split that survive until the pointwise RHS
+    R AU[2][H], BU[2][H], AV[2][H], BV[2][H], CF[2][H];
+    cx<R> c[P];                                   // packed Coeff1 of both stages
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
+        // (1) doubled spectrum + mode-2 fix (ICG.cpp:797-800), tau-derivative (803)
+        cx<R> z[2][P];
         const R fix = bc[s];
 #pragma unroll
         for (int h = 0; h < H; ++h) {
@@ -85,43 +98,42 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
                 hi = Y;
             }
             if (q == 2) lo.x = -fix;
-            z[s][0][h] = lo;
-            z[s][0][h + H] = hi;
+            z[0][h] = lo;
+            z[0][h + H] = hi;
         }
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const cx<R> v = z[s][0][p];
-            z[s][1][p] = {wk[p] * v.y, -(wk[p] * v.x)};
+            const int k = t + T * p;
+            const R w = (k == N / 2) ? (R)0 : (R)wavenumber<N>(k) * K.k0;
+            const cx<R> v = z[0][p];
+            z[1][p] = {w * v.y, -(w * v.x)};
+        }
+        // (2) back to tau (804-805)
+        fft_block<N, T, -1, 2>(z, xb, tw, t);
+        // (3) half-period split (814-827), IA2 coefficients (834-836), RHS products
+        const R X = xs[s];
+        const R inv_x = (R)1 / X;
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            const cx<R> a = z[0][h], b = z[0][h + H];
+            const cx<R> a2 = z[1][h], b2 = z[1][h + H];
+            const R U = half * (a.x - b.x), V = half * (a.y - b.y), F = half * (a.y + b.y);
+            const R dU = half * (a2.x - b2.x), dV = half * (a2.y - b2.y);
+            const R C1 = -(((X + F) * U * U + (X - F) * V * V) * K.c3 / ((R)8 * X) + K.dm3);
+            if (s == 0) c[h].x = C1; else c[h].y = C1;
+            const R two_x = (R)2 * X;
+            const R inv1 = (R)1 / (two_x * (F + X));
+            const R inv2 = (R)1 / (two_x * (F - X));
+            const R FD = F * K.dm2 * (U + V);
+            AU[s][h] = (FD - two_x * dU) * inv1;
+            BU[s][h] = K.two_dm3 * F * U * inv1;
+            AV[s][h] = (FD + two_x * dV) * inv2;
+            BV[s][h] = K.two_dm3 * F * V * inv2;
+            CF[s][h] = K.dm3 * F * inv_x;
         }
     }
-    // (2) back to tau (804-805)
-    fft_block<N, T, -1, 2>(z[0], xb, tw, t);
-    fft_block<N, T, -1, 2>(z[1], xb, tw, t);
-
-    // (3) split by half-period symmetry (814-827) and IA2 coefficients (834-836)
-    R U[2][H], V[2][H], F[2][H], dU[2][H], dV[2][H], dF[2][H];
-    cx<R> c[P];
 #pragma unroll
-    for (int h = 0; h < H; ++h) {
-        R C1[2];
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const cx<R> a = z[s][0][h], b = z[s][0][h + H];
-            const cx<R> a2 = z[s][1][h], b2 = z[s][1][h + H];
-            U[s][h] = half * (a.x - b.x);
-            V[s][h] = half * (a.y - b.y);
-            F[s][h] = half * (a.y + b.y);
-            dU[s][h] = half * (a2.x - b2.x);
-            dV[s][h] = half * (a2.y - b2.y);
-            dF[s][h] = half * (a2.y + b2.y);
-            const R X = xs[s];
-            C1[s] = -(((X + F[s][h]) * U[s][h] * U[s][h] + (X - F[s][h]) * V[s][h] * V[s][h]) * K.c3 /
-                          ((R)8 * X) +
-                      K.dm3);
-        }
-        c[h] = {C1[0], C1[1]};
-        c[h + H] = c[h];
-    }
+    for (int h = 0; h < H; ++h) c[h + H] = c[h];
 
     // (4) solveInhom step 1 for both stages: F' = Coeff1 (264-270)
     fft1<N, T, +1>(c, xb, tw, t);
@@ -133,23 +145,23 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {      // lamIntegrate, lambda = 0: G = f / (-i m k0)
+        const R im = invm[t + T * p] * K.inv_k0;
         const cx<R> v = c[p];
-        c[p] = {-(v.y * imk[p]), v.x * imk[p]};
+        c[p] = {-(v.y * im), v.x * im};
     }
     fft1<N, T, -1>(c, xb, tw, t);
-    R Fr[2][P];
+    // (5) h = -g e^{F} (276-279), forward (283); keep e^{F}
+    R E[2][P];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-        Fr[0][p] = c[p].x;
-        Fr[1][p] = c[p].y;
+        E[0][p] = exp(c[p].x);
+        E[1][p] = exp(c[p].y);
+        c[p] = {K.mg * E[0][p], K.mg * E[1][p]};
     }
-    // (5) h = -g e^{F} (276-279), forward (283)
-#pragma unroll
-    for (int p = 0; p < P; ++p) c[p] = {K.mg * exp(Fr[0][p]), K.mg * exp(Fr[1][p])};
     fft1<N, T, +1>(c, xb, tw, t);
 #pragma unroll
     for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
-    // (6) unpack the two Hermitian spectra, lamIntegrate with lambda_s (284)
+    // (6) separate the two Hermitian spectra, lamIntegrate with lambda_s (284)
 #pragma unroll
     for (int p = 0; p < P; ++p) xb[t + T * p] = c[p];
     __syncthreads();
@@ -174,10 +186,9 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
             if (k == 0) {
                 G[s] = (lam != (R)0) ? cx<R>{hs.x / lam, hs.y / lam} : cx<R>{(R)0, (R)0};
             } else if (k == N / 2) {
-                const R fac = lam / (lam * lam + K.q2);
-                G[s] = cscale(hs, fac);
+                G[s] = cscale(hs, lam / (lam * lam + K.q2));
             } else {
-                const R w = wk[p];
+                const R w = (R)wavenumber<N>(k) * K.k0;
                 const R den = lam * lam + w * w;
                 if (sqrt(den) < (R)1e-14) {
                     G[s] = {(R)0, (R)0};
@@ -190,33 +201,19 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         c[p] = {G[0].x - G[1].y, G[0].y + G[1].x};
     }
     fft1<N, T, -1>(c, xb, tw, t);
-    R IA2[2][P];
-#pragma unroll
-    for (int p = 0; p < P; ++p) {                   // X = e^{-F} Y (288-291)
-        IA2[0][p] = c[p].x * exp(-Fr[0][p]);
-        IA2[1][p] = c[p].y * exp(-Fr[1][p]);
-    }
 
-    // (7) pointwise RHS (ODEStepper.cpp:109-122)
+    // (7) pointwise RHS with r = 1/IA2 = e^{F}/X (288-291; ODEStepper.cpp:109-122)
     cx<R> yp[2][P];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const R x = xs[s];
-        const R two_x = (R)2 * x;
+    for (int p = 0; p < P; ++p) {
+        const int ph = (p < H) ? p : p - H;
+        const R sg = (p < H) ? (R)1 : (R)-1;
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const int ph = (p < H) ? p : p - H;
-            const R sg = (p < H) ? (R)1 : (R)-1;
-            const R Uv = sg * U[s][ph], Vv = sg * V[s][ph], Fv = F[s][ph];
-            const R dUv = sg * dU[s][ph], dVv = sg * dV[s][ph];
-            const R ia = IA2[s][p];
-            const R cc = K.two_dm3 / ia - K.dm2;
-            const R dUdx = (Fv * (K.dm2 * Vv - cc * Uv) - two_x * dUv) / (two_x * (Fv + x));
-            const R dVdx = (Fv * (K.dm2 * Uv - cc * Vv) + two_x * dVv) / (two_x * (Fv - x));
-            const R dFdx = K.dm3 * Fv * ((R)1 - ia) / (ia * x);
-            yp[s][p] = {dUdx, dVdx + dFdx};
+        for (int s = 0; s < 2; ++s) {
+            const R r = E[s][p] / (s ? c[p].y : c[p].x);
+            const R cf = CF[s][ph];
+            yp[s][p] = {sg * (AU[s][ph] - BU[s][ph] * r), sg * (AV[s][ph] - BV[s][ph] * r) + cf * (r - (R)1)};
         }
-        (void)dF;
     }
     // (8) FieldsToStateVector (766-778) and Re dY[2] = 0 (ODEStepper.cpp:128)
     fft_block<N, T, +1, 2>(yp, xb, tw, t);
@@ -230,6 +227,23 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
             if (q == 2) v.x = (R)0;
             f[s][h] = v;
         }
+}
+
+template <int N, typename R>
+__device__ __forceinline__ RhsConst<R> make_const(double Dim, double delta)
+{
+    const double k0d = 2.0 * 3.14159265358979323846 / delta;
+    RhsConst<R> K;
+    K.k0 = (R)k0d;
+    K.inv_k0 = (R)(1.0 / k0d);
+    K.dm2 = (R)(Dim - 2.0);
+    K.dm3 = (R)(Dim - 3.0);
+    K.c3 = (R)((Dim - 2.0) * (Dim - 2.0) * (Dim - 2.0));
+    K.two_dm3 = (R)(2.0 * (Dim - 3.0));
+    K.mg = (R)(-(Dim - 3.0));
+    const double q = k0d * (double)N / 2.0;
+    K.q2 = (R)(q * q);
+    return K;
 }
 
 template <int N, int T, typename R>
@@ -248,30 +262,10 @@ __global__ __launch_bounds__(T) void k_chain(ChainParams<R> a)
     if (s_begin >= s_end) return;
 
     cx<R> *Yg = a.Y + (size_t)ch * M;
-    cx<R> y[H], yk[2][H], f[2][H];
+    cx<R> y[H], yk[2][H], yn[H];
 #pragma unroll
     for (int h = 0; h < H; ++h) y[h] = Yg[t + T * h];
-
-    const double k0d = 2.0 * 3.14159265358979323846 / a.delta[ch];
-    const R k0 = (R)k0d;
-    RhsConst<R> K;
-    K.dm2 = (R)(a.Dim - 2.0);
-    K.dm3 = (R)(a.Dim - 3.0);
-    K.c3 = (R)((a.Dim - 2.0) * (a.Dim - 2.0) * (a.Dim - 2.0));
-    K.two_dm3 = (R)(2.0 * (a.Dim - 3.0));
-    K.mg = (R)(-(a.Dim - 3.0));
-    {
-        const double q = k0d * (double)N / 2.0;
-        K.q2 = (R)(q * q);
-    }
-    R wk[P], imk[P];
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        const int k = t + T * p;
-        const int m = wavenumber<N>(k);
-        wk[p] = (k == N / 2) ? (R)0 : (R)m * k0;
-        imk[p] = (k == 0 || k == N / 2) ? (R)0 : (R)1 / ((R)m * k0);
-    }
+    const RhsConst<R> K = make_const<N, R>(a.Dim, a.delta[ch]);
 
     const int dir = a.dir[ch];
     int idx = a.start[ch] + dir * s_begin;
@@ -280,7 +274,6 @@ __global__ __launch_bounds__(T) void k_chain(ChainParams<R> a)
     for (int step = s_begin; step < s_end; ++step, idx += dir) {
         const double xin = a.grid[idx], xout = a.grid[idx + dir];
         const double dxd = xout - xin;
-        const R dx = (R)dxd;
         const R xs[2] = {(R)(xin + dxd * a.c0), (R)(xin + dxd * a.c1)};
         const R A00 = (R)(dxd * a.a00), A01 = (R)(dxd * a.a01), A10 = (R)(dxd * a.a10), A11 = (R)(dxd * a.a11);
         const R B0 = (R)(dxd * a.b0), B1 = (R)(dxd * a.b1);
@@ -291,26 +284,33 @@ __global__ __launch_bounds__(T) void k_chain(ChainParams<R> a)
         bool conv = false;
         int its = 0;
         for (its = 0; its < a.maxIts; ++its) {
-            rhs_pair<N, T, R>(yk, f, xs, K, wk, imk, xb, bc, a.tw, t);
+            cx<R> f[2][H];
+            rhs_pair<N, T, R>(yk, f, xs, K, a.invm, xb, bc, a.tw, t);
             R loc = (R)0;
 #pragma unroll
             for (int h = 0; h < H; ++h) {
-                cx<R> n0 = y[h], n1 = y[h];
+                // ODEStepper.cpp:241-259 (stage update, same operation order)
+                cx<R> n0 = y[h], n1 = y[h], nn = y[h];
                 n0.x += A00 * f[0][h].x; n0.y += A00 * f[0][h].y;
                 n0.x += A01 * f[1][h].x; n0.y += A01 * f[1][h].y;
                 n1.x += A10 * f[0][h].x; n1.y += A10 * f[0][h].y;
                 n1.x += A11 * f[1][h].x; n1.y += A11 * f[1][h].y;
+                // ODEStepper.cpp:279-285 (final combination, in case this is the last iterate)
+                nn.x += B0 * f[0][h].x; nn.y += B0 * f[0][h].y;
+                nn.x += B1 * f[1][h].x; nn.y += B1 * f[1][h].y;
                 const R d0x = yk[0][h].x - n0.x, d0y = yk[0][h].y - n0.y;
                 const R d1x = yk[1][h].x - n1.x, d1y = yk[1][h].y - n1.y;
                 loc += d0x * d0x + d0y * d0y + d1x * d1x + d1y * d1y;
                 yk[0][h] = n0;
                 yk[1][h] = n1;
+                yn[h] = nn;
             }
             if (t == T - 2) {
                 bc[0] = yk[0][H - 1].x;
                 bc[1] = yk[1][H - 1].x;
             }
-            const R norm2 = block_sum<T, R>(loc, red, t);      // barrier publishes bc[0..1]
+            const R norm2 = block_sum<T, R>(loc, red, t);
+            __syncthreads();                                  // publishes bc[0..1]
             const double rms = sqrt((double)norm2 / (double)(N * 2));
             double tol = a.precision;
             if (its > a.maxIts / 2) tol = a.precision * 10.0 * (2.0 * (double)its / (double)a.maxIts - 0.5);
@@ -326,11 +326,7 @@ __global__ __launch_bounds__(T) void k_chain(ChainParams<R> a)
             break;
         }
 #pragma unroll
-        for (int h = 0; h < H; ++h) {                   // ODEStepper.cpp:279-285
-            y[h].x += B0 * f[0][h].x; y[h].y += B0 * f[0][h].y;
-            y[h].x += B1 * f[1][h].x; y[h].y += B1 * f[1][h].y;
-        }
-        __syncthreads();
+        for (int h = 0; h < H; ++h) y[h] = yn[h];
     }
 #pragma unroll
     for (int h = 0; h < H; ++h) Yg[t + T * h] = y[h];
@@ -341,10 +337,10 @@ __global__ __launch_bounds__(T) void k_chain(ChainParams<R> a)
 }
 
 // computeDerivatives on pairs of states (unit entry point).  Y, dY: [2*npairs][M];
-// xs, delta: [2*npairs].
+// xs, delta: [2*npairs]; both members of a pair share Delta.
 template <int N, int T, typename R>
 __global__ __launch_bounds__(T) void k_rhs_pairs(const cx<R> *Y, const double *xs, const double *delta,
-                                                 cx<R> *dY, const cx<R> *tw, double Dim)
+                                                 cx<R> *dY, const cx<R> *tw, const R *invm, double Dim)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
     __shared__ cx<R> xb[2 * N];
@@ -361,27 +357,9 @@ __global__ __launch_bounds__(T) void k_rhs_pairs(const cx<R> *Y, const double *x
         bc[1] = yk[1][H - 1].x;
     }
     __syncthreads();
-    // both states of a pair must share Delta (the stages of one step do)
-    const double k0d = 2.0 * 3.14159265358979323846 / delta[2 * pr];
-    const R k0 = (R)k0d;
-    RhsConst<R> K;
-    K.dm2 = (R)(Dim - 2.0);
-    K.dm3 = (R)(Dim - 3.0);
-    K.c3 = (R)((Dim - 2.0) * (Dim - 2.0) * (Dim - 2.0));
-    K.two_dm3 = (R)(2.0 * (Dim - 3.0));
-    K.mg = (R)(-(Dim - 3.0));
-    const double q = k0d * (double)N / 2.0;
-    K.q2 = (R)(q * q);
-    R wk[P], imk[P];
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        const int k = t + T * p;
-        const int m = wavenumber<N>(k);
-        wk[p] = (k == N / 2) ? (R)0 : (R)m * k0;
-        imk[p] = (k == 0 || k == N / 2) ? (R)0 : (R)1 / ((R)m * k0);
-    }
+    const RhsConst<R> K = make_const<N, R>(Dim, delta[2 * pr]);
     const R xsr[2] = {(R)xs[2 * pr], (R)xs[2 * pr + 1]};
-    rhs_pair<N, T, R>(yk, f, xsr, K, wk, imk, xb, bc, tw, t);
+    rhs_pair<N, T, R>(yk, f, xsr, K, invm, xb, bc, tw, t);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll

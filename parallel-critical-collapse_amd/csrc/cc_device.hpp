@@ -128,11 +128,25 @@ __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const
         const int k = j & (NS - 1);
         cx<R> w[RAD];
         if constexpr (NS > 1) {
+            // w^r for r < RAD, w = e^{SIGN 2 pi i k / (NS RAD)}.  Radix <= 4: one
+            // table load and products (w^2 = w w, w^3 = w w^2); larger radices
+            // load every power.
+            constexpr int STRIDE = N / (NS * RAD);
+            if constexpr (RAD <= 4) {
+                cx<R> w1 = tw[k * STRIDE];
+                if constexpr (SIGN < 0) w1.y = -w1.y;
+                w[1] = w1;
+                if constexpr (RAD == 4) {
+                    w[2] = cmul(w1, w1);
+                    w[3] = cmul(w1, w[2]);
+                }
+            } else {
 #pragma unroll
-            for (int r = 1; r < RAD; ++r) {
-                cx<R> tr = tw[r * k * (N / (NS * RAD))];
-                if constexpr (SIGN < 0) tr.y = -tr.y;
-                w[r] = tr;
+                for (int r = 1; r < RAD; ++r) {
+                    cx<R> tr = tw[r * k * STRIDE];
+                    if constexpr (SIGN < 0) tr.y = -tr.y;
+                    w[r] = tr;
+                }
             }
         }
 #pragma unroll

@@ -178,7 +178,7 @@ __device__ void unpack_fields(Spec<N, T> &S, const double *Z, int eps_col, doubl
         if (i == N / 2 + 2) return 0.0;                 // NewtonSolver.cpp:408-409
         return Z[i] + ((i == eps_col) ? eps : 0.0);
     };
-    cx<double> A[P], E[P];
+    cx<double> A1[P], A2[P], E[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         const int k = S.kidx(p);
@@ -204,16 +204,19 @@ __device__ void unpack_fields(Spec<N, T> &S, const double *Z, int eps_col, doubl
                 ev = {zv(N / 2 + kk), -zv(N / 2 + kk + 1)};
             }
         }
-        // Up and psic share one transform (both spectra are Hermitian)
-        A[p] = {o1.x - o2.y, o1.y + o2.x};
+        A1[p] = o1;
+        A2[p] = o2;
         E[p] = ev;
     }
-    S.bwd(A);
+    // one transform per field, as the reference: a column that perturbs one
+    // field must leave the other two bit-identical
+    S.bwd(A1);
+    S.bwd(A2);
     S.bwd(E);
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-        Up[p] = A[p].x;
-        psic[p] = A[p].y;
+        Up[p] = A1[p].x;
+        psic[p] = A2[p].x;
         fc[p] = E[p].x;
     }
 }

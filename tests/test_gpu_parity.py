@@ -180,12 +180,16 @@ def test_newton_step_matches_oracle_c1(c1_reference):
 
 
 def test_solve_golden_10x10():
-    """test/test_lapack.cpp:26-74: solution 1..10 within 1e-12."""
+    """The system of test/test_lapack.cpp:26-74 (whose written solution 1..10 is
+    not its solution; see test_oracle.test_dgesv_golden_10x10): same answer as the
+    oracle's dgesv within 1e-12."""
     g = golden("lapack_10.json")
+    A, b = np.array(g["A"]), np.array(g["b"])
     ctx = cc.Context(cc.Problem.make(Ntau=128, NLeft=10, NRight=10))
-    x, info, _, st = ctx.solve(np.array(g["A"]), np.array(g["b"]))
-    assert st == 0 and info == 0
-    assert np.max(np.abs(x - np.array(g["x"]))) < 1e-12
+    x, info, _, st = ctx.solve(A, b)
+    ref, info_ref, _, _ = O.dgesv(A, b)
+    assert st == 0 and info == 0 == info_ref
+    assert np.max(np.abs(x - ref)) < 1e-12
 
 
 @pytest.mark.parametrize("n", [7, 64, 96, 200, 768])
