@@ -73,6 +73,9 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     constexpr int P = N / T, H = P / 2, M = N / 2;
     const R half = (R)0.5;
     const R invN = (R)1 / (R)N;
+    // Everything derived from the thread index is recomputed per call: held
+    // across the fixed-point loop it would cost more registers than it saves.
+    asm volatile("" : "+v"(t));
 
     // per-stage products of the split that survive until the pointwise RHS
     R AU[2][H], BU[2][H], AV[2][H], BV[2][H], CF[2][H];
@@ -247,7 +250,7 @@ __device__ __forceinline__ RhsConst<R> make_const(double Dim, double delta)
 }
 
 template <int N, int T, typename R>
-__global__ __launch_bounds__(T) void k_chain(ChainParams<R> a)
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 2 : 3))) void k_chain(ChainParams<R> a)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
     __shared__ cx<R> xb[2 * N];
@@ -262,9 +265,13 @@ __global__ __launch_bounds__(T) void k_chain(ChainParams<R> a)
     if (s_begin >= s_end) return;
 
     cx<R> *Yg = a.Y + (size_t)ch * M;
-    cx<R> y[H], yk[2][H], yn[H];
+    // The step-start state y and the candidate next state y + dx sum b_i f_i
+    // are read once per iteration: they live in LDS (thread-private slots),
+    // the stage iterates in registers.
+    __shared__ cx<R> ys[2][M];
+    cx<R> yk[2][H];
 #pragma unroll
-    for (int h = 0; h < H; ++h) y[h] = Yg[t + T * h];
+    for (int h = 0; h < H; ++h) ys[0][t + T * h] = Yg[t + T * h];
     const RhsConst<R> K = make_const<N, R>(a.Dim, a.delta[ch]);
 
     const int dir = a.dir[ch];
@@ -278,8 +285,8 @@ __global__ __launch_bounds__(T) void k_chain(ChainParams<R> a)
         const R A00 = (R)(dxd * a.a00), A01 = (R)(dxd * a.a01), A10 = (R)(dxd * a.a10), A11 = (R)(dxd * a.a11);
         const R B0 = (R)(dxd * a.b0), B1 = (R)(dxd * a.b1);
 #pragma unroll
-        for (int h = 0; h < H; ++h) yk[0][h] = yk[1][h] = y[h];
-        if (t == T - 2) bc[0] = bc[1] = y[H - 1].x;   // Re of mode M-2 for the fix
+        for (int h = 0; h < H; ++h) yk[0][h] = yk[1][h] = ys[0][t + T * h];
+        if (t == T - 2) bc[0] = bc[1] = yk[0][H - 1].x;   // Re of mode M-2 for the fix
         __syncthreads();
         bool conv = false;
         int its = 0;
@@ -290,7 +297,8 @@ __global__ __launch_bounds__(T) void k_chain(ChainParams<R> a)
 #pragma unroll
             for (int h = 0; h < H; ++h) {
                 // ODEStepper.cpp:241-259 (stage update, same operation order)
-                cx<R> n0 = y[h], n1 = y[h], nn = y[h];
+                const cx<R> y = ys[0][t + T * h];
+                cx<R> n0 = y, n1 = y, nn = y;
                 n0.x += A00 * f[0][h].x; n0.y += A00 * f[0][h].y;
                 n0.x += A01 * f[1][h].x; n0.y += A01 * f[1][h].y;
                 n1.x += A10 * f[0][h].x; n1.y += A10 * f[0][h].y;
@@ -303,7 +311,7 @@ __global__ __launch_bounds__(T) void k_chain(ChainParams<R> a)
                 loc += d0x * d0x + d0y * d0y + d1x * d1x + d1y * d1y;
                 yk[0][h] = n0;
                 yk[1][h] = n1;
-                yn[h] = nn;
+                ys[1][t + T * h] = nn;
             }
             if (t == T - 2) {
                 bc[0] = yk[0][H - 1].x;
@@ -326,10 +334,10 @@ __global__ __launch_bounds__(T) void k_chain(ChainParams<R> a)
             break;
         }
 #pragma unroll
-        for (int h = 0; h < H; ++h) y[h] = yn[h];
+        for (int h = 0; h < H; ++h) ys[0][t + T * h] = ys[1][t + T * h];
     }
 #pragma unroll
-    for (int h = 0; h < H; ++h) Yg[t + T * h] = y[h];
+    for (int h = 0; h < H; ++h) Yg[t + T * h] = ys[0][t + T * h];
     if (t == 0) {
         a.its[ch] += its_sum;
         if (st) a.status[ch] = st;

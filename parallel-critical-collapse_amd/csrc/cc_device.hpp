@@ -125,7 +125,11 @@ __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int j = t + T * b;
-        const int k = j & (NS - 1);
+        int k = j & (NS - 1);
+        // keep the twiddle computation inside the loop that calls the
+        // transform: hoisting it out of the fixed-point loop costs far more
+        // registers than the loads cost bandwidth
+        asm volatile("" : "+v"(k));
         cx<R> w[RAD];
         if constexpr (NS > 1) {
             // w^r for r < RAD, w = e^{SIGN 2 pi i k / (NS RAD)}.  Radix <= 4: one

@@ -105,8 +105,16 @@ __global__ void k_lu_trsm(R *A, int n, int k0)
 }
 
 // MFMA 16x16x4 wrappers.  One wave computes a 16x16 tile D += A(16x4) B(4x16).
-// Lane l supplies A[l%16][l/16] and B[l/16][l%16]; it receives D rows
-// 4*(l/16)+r, column l%16, r = 0..3.
+// Lane l supplies A[l%16][l/16] and B[l/16][l%16] and receives four elements
+// of column l%16 of D: rows 4*(l/16)+r for the f32 instruction (one group of
+// four consecutive rows per 16-lane block) and rows l/16 + 4*r for the f64
+// instruction (row groups of size one interleaved across the blocks).
+template <typename R>
+__device__ __forceinline__ int mfma16_row(int lane, int r)
+{
+    if constexpr (sizeof(R) == 8) return (lane >> 4) + 4 * r;
+    else return 4 * (lane >> 4) + r;
+}
 __device__ __forceinline__ void mfma16(double a, double b, double (&acc)[4])
 {
     typedef double v4d __attribute__((ext_vector_type(4)));
@@ -173,7 +181,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(R *A, int n, int k0)
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gr = r0 + wr + a * 16 + 4 * (lane >> 4) + r;
+                const int gr = r0 + wr + a * 16 + mfma16_row<R>(lane, r);
                 const int gc = c0 + wc + b * 16 + (lane & 15);
                 if (gr < n && gc < n) A[(size_t)gr * n + gc] -= acc[a][b][r];
             }

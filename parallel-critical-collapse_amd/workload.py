@@ -91,7 +91,9 @@ def config(index: int) -> Shape:
     if index == 2:
         return Shape(Ntau=512, X=6000, seed=SEED_BASE + 2)
     if index == 3:
-        return Shape(Ntau=1024, X=8000, seed=SEED_BASE + 3)
+        # scale 1 makes the FP64 oracle's right-side IRK iteration fail near
+        # x = 0.7; 10^-1 is the first power of ten it survives (k = 5.1)
+        return Shape(Ntau=1024, X=8000, seed=SEED_BASE + 3, scale=0.1)
     if index == 5:
         return Shape(Ntau=2048, X=12000, seed=SEED_BASE + 5)
     raise ValueError(index)
