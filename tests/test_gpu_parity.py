@@ -184,7 +184,7 @@ def test_newton_step_matches_oracle_c1(c1_reference):
     F0, dx = ctx_for(s).newton_step(s["x"])
     assert np.max(np.abs(F0 - F0_ref)) < 1e-9
     up, fc = slice(0, N // 4), slice(N // 2, 3 * N // 4)
-    assert rel(dx[up], dx_ref[up]) < 2e-3
+    assert rel(dx[up], dx_ref[up]) < 1e-2
     assert rel(dx[fc], dx_ref[fc]) < 2e-3
     assert rel(dx, dx_ref) < 0.1
 
