@@ -20,6 +20,7 @@
 
 #include "../../include/cc_gpu.h"
 #include "cc_kernels.hpp"
+#include "cc_tangent.hpp"
 
 using ccd::cx;
 
@@ -60,7 +61,17 @@ struct cc_ctx {
     float *A32 = nullptr;                // n x n LU work (FP32)
     int32_t *ipiv = nullptr, *info = nullptr, *cols = nullptr;
     ColumnBuffers main;                  // the Jacobian columns
-    ColumnBuffers base;                  // the concurrent FP64 base shoot of the mixed policy
+    ColumnBuffers base;                  // FP64 base (+ Delta column) of the mixed policy
+    // tangent-linear columns of the mixed policy
+    float *rec = nullptr;                // base-trajectory record, X steps
+    int64_t *rec_base_b = nullptr;       // record slots of the FP64 base launch's chains
+    int64_t *rec_base_t = nullptr;       // record slots of the tangent chains
+    cx<float> *dY = nullptr;             // [n][M] tangent states
+    int32_t *t_start = nullptr, *t_nsteps = nullptr, *t_dir = nullptr, *t_status = nullptr, *t_fail = nullptr;
+    int32_t *t_cols = nullptr, *t_lcols = nullptr, *t_rcols = nullptr, *t_lch = nullptr, *t_rch = nullptr;
+    int32_t *t_lidx = nullptr, *t_ridx = nullptr;
+    int64_t *t_its = nullptr;
+    double tangent_rtol = 1e-6, tangent_h = 1e-5;
     int maxcols = 0, maxchains = 0;
     std::vector<void *> allocs;
     // chain-kernel timing of the main stream
@@ -182,7 +193,7 @@ hipError_t up(const std::vector<Tp> &v, Tp *d, hipStream_t s)
 // per-chain status and per-column mismatch status in B.
 template <int N, typename R>
 cc_status launch_columns(cc_ctx *c, const Plan &P, const ColumnBuffers &B, const double *d_x, double eps,
-                         double precision, hipStream_t s)
+                         double precision, hipStream_t s, float *rec = nullptr, const int64_t *rec_base = nullptr)
 {
     constexpr int T = tpb(N);
     const int nch = P.nchains;
@@ -236,8 +247,10 @@ cc_status launch_columns(cc_ctx *c, const Plan &P, const ColumnBuffers &B, const
     a.b1 = c->b[1];
     a.c0 = c->c[0];
     a.c1 = c->c[1];
+    a.rec = rec;
+    a.rec_base = rec_base;
     const int maxsteps = *std::max_element(P.nsteps.begin(), P.nsteps.end());
-    const bool timed = c->profile && s == c->s0;
+    const bool timed = c->profile && s == c->s0 && rec == nullptr;
     if (timed) HIPCHK(c, hipEventRecord(c->tk0, s));
     int launches = 0;
     for (int s0 = 0; s0 < maxsteps; s0 += kStepsPerLaunch) {
@@ -386,6 +399,137 @@ std::vector<int32_t> jacobian_cols(const cc_ctx *c, bool with_base)
     return cols;
 }
 
+// Mixed-policy Jacobian on stream s0.  (1) FP64 launch of the base column
+// (recording its trajectory) and of the Delta column; F(x) = row 0 of
+// base.F, the Delta column is a forward difference in FP64 (its direction
+// changes k0 = 2 pi / Delta inside every spectral operator).  (2) FP32
+// tangent-linear columns for every other slot, one chain each: Up slots only
+// move the right boundary data, psic/fc slots only the left, so the other
+// side's tangent is exactly zero.
+template <int N>
+cc_status mixed_jacobian_n(cc_ctx *c, const double *d_x, std::vector<int32_t> *tstatus_out, std::vector<int32_t> *tcols_out)
+{
+    constexpr int T = tpb(N);
+    const int n = c->n, dslot = N / 2 + 2;
+    hipStream_t s = c->s0;
+    // (1) FP64: base + Delta column
+    Plan Pb = make_plan(c, std::vector<int32_t>{-1, dslot}, false);
+    const std::vector<int64_t> rb = {0, (int64_t)c->iM, -1, -1};      // chains 0,1 = base L,R
+    HIPCHK(c, hipMemcpyAsync(c->rec_base_b, rb.data(), sizeof(int64_t) * rb.size(), hipMemcpyHostToDevice, s));
+    cc_status st = launch_columns<N, double>(c, Pb, c->base, d_x, c->p.EpsNewton, c->p.PrecisionIRK, s, c->rec,
+                                             c->rec_base_b);
+    if (st != CC_OK) return st;
+    {
+        HIPCHK(c, up(Pb.cols, c->cols, s));
+        hipLaunchKernelGGL(ccd::k_fd, dim3(1, n), dim3(128), 0, s, c->base.F, c->cols, 2, 0, n, c->p.EpsNewton, c->J);
+        HIPCHK(c, hipGetLastError());
+    }
+    st = collect_status(c, Pb, c->base, s, nullptr, true);
+    if (st != CC_OK) return st;
+    // (2) FP32 tangent columns
+    std::vector<int32_t> cols, lcols, lch, rcols, rch, lidx, ridx, start, nsteps, dir;
+    std::vector<int64_t> rbt;
+    for (int i = 0; i < n; ++i) {
+        if (i == dslot) continue;
+        const int id = (int)start.size();
+        const bool left = i >= N / 4;
+        cols.push_back(i);
+        if (left) {
+            lcols.push_back(i); lch.push_back(id); lidx.push_back(id); ridx.push_back(-1);
+            start.push_back(0); nsteps.push_back(c->iM); dir.push_back(1); rbt.push_back(0);
+        } else {
+            rcols.push_back(i); rch.push_back(id); lidx.push_back(-1); ridx.push_back(id);
+            start.push_back(c->iR); nsteps.push_back(c->iR - c->iM); dir.push_back(-1); rbt.push_back(c->iM);
+        }
+    }
+    const int nch = (int)cols.size();
+    HIPCHK(c, up(cols, c->t_cols, s));
+    HIPCHK(c, up(lcols, c->t_lcols, s));
+    HIPCHK(c, up(lch, c->t_lch, s));
+    HIPCHK(c, up(rcols, c->t_rcols, s));
+    HIPCHK(c, up(rch, c->t_rch, s));
+    HIPCHK(c, up(lidx, c->t_lidx, s));
+    HIPCHK(c, up(ridx, c->t_ridx, s));
+    HIPCHK(c, up(start, c->t_start, s));
+    HIPCHK(c, up(nsteps, c->t_nsteps, s));
+    HIPCHK(c, up(dir, c->t_dir, s));
+    HIPCHK(c, hipMemcpyAsync(c->rec_base_t, rbt.data(), sizeof(int64_t) * nch, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemsetAsync(c->t_status, 0, sizeof(int32_t) * nch, s));
+    HIPCHK(c, hipMemsetAsync(c->t_its, 0, sizeof(int64_t) * nch, s));
+    if (!lcols.empty())
+        hipLaunchKernelGGL((ccd::k_boundary_tangent<N, T, true>), dim3((unsigned)lcols.size()), dim3(T), 0, s, d_x,
+                           c->t_lcols, c->t_lch, c->tangent_h, c->p.XLeft, c->p.Dim, c->tw64, c->dY);
+    if (!rcols.empty())
+        hipLaunchKernelGGL((ccd::k_boundary_tangent<N, T, false>), dim3((unsigned)rcols.size()), dim3(T), 0, s, d_x,
+                           c->t_rcols, c->t_rch, c->tangent_h, c->p.XRight, c->p.Dim, c->tw64, c->dY);
+    HIPCHK(c, hipGetLastError());
+    ccd::TangentParams a{};
+    a.dY = c->dY;
+    a.rec = c->rec;
+    a.rec_base = c->rec_base_t;
+    a.start = c->t_start;
+    a.nsteps = c->t_nsteps;
+    a.dir = c->t_dir;
+    a.delta = c->base.delta;          // [0]: the base's Delta, shared by every tangent chain
+    a.grid = c->grid;
+    a.status = c->t_status;
+    a.fail_idx = c->t_fail;
+    a.its = c->t_its;
+    a.tw = c->tw32;
+    a.invm = c->invm32;
+    a.maxIts = c->p.MaxIterIRK;
+    a.Dim = c->p.Dim;
+    a.rtol = c->tangent_rtol;
+    a.a00 = c->a[0][0]; a.a01 = c->a[0][1]; a.a10 = c->a[1][0]; a.a11 = c->a[1][1];
+    a.b0 = c->b[0]; a.b1 = c->b[1]; a.c0 = c->c[0]; a.c1 = c->c[1];
+    const int maxsteps = std::max(c->iM, c->iR - c->iM);
+    if (c->profile) HIPCHK(c, hipEventRecord(c->tk0, s));
+    int launches = 0;
+    for (int s0 = 0; s0 < maxsteps; s0 += kStepsPerLaunch) {
+        a.step_begin = s0;
+        a.step_end = std::min(maxsteps, s0 + kStepsPerLaunch);
+        hipLaunchKernelGGL((ccd::k_tangent<N, T>), dim3(nch), dim3(T), 0, s, a);
+        ++launches;
+    }
+    if (c->profile) {
+        HIPCHK(c, hipEventRecord(c->tk1, s));
+        c->prof_launches = launches;
+        c->prof_pending = true;
+    }
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL((ccd::k_tangent_column<N, T>), dim3(nch), dim3(T), 0, s, c->dY, c->t_cols, c->t_lidx,
+                       c->t_ridx, c->tw64, c->J);
+    HIPCHK(c, hipGetLastError());
+    std::vector<int32_t> tst(nch);
+    std::vector<int64_t> tits(nch);
+    HIPCHK(c, hipMemcpyAsync(tst.data(), c->t_status, sizeof(int32_t) * nch, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(tits.data(), c->t_its, sizeof(int64_t) * nch, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    int64_t itsum = 0, steps = 0;
+    cc_status worst = CC_OK;
+    for (int b = 0; b < nch; ++b) {
+        itsum += tits[b];
+        steps += nsteps[b];
+        if (tst[b]) worst = CC_IRK_NOCONV;
+    }
+    c->counters[0] += steps;
+    c->counters[1] += itsum;
+    c->counters[2] += nch;
+    c->prof_its = itsum;
+    c->prof_steps = steps;
+    if (tstatus_out) *tstatus_out = tst;
+    if (tcols_out) *tcols_out = cols;
+    if (worst != CC_OK) c->err = "tangent-linear IRK iteration did not converge";
+    return worst;
+}
+
+cc_status mixed_jacobian(cc_ctx *c, const double *d_x, std::vector<int32_t> *tst, std::vector<int32_t> *tcols)
+{
+    cc_status st = CC_OK;
+    dispatch_n(c->N, [&](auto Nc) { st = mixed_jacobian_n<decltype(Nc)::value>(c, d_x, tst, tcols); });
+    return st;
+}
+
 // The Newton step on device arrays.  FP64: one launch of n+1 columns.
 // Mixed: FP64 base shoot on s1 concurrently with the FP32 columns on s0.
 cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double *d_dx, bool sync_status)
@@ -394,43 +538,28 @@ cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double 
     const bool mixed = c->p.precision == CC_MIXED;
     const bool reuse = c->p.reuse_sides != 0;
     c->counters[0] = c->counters[1] = c->counters[2] = 0;
-    Plan Pm = make_plan(c, jacobian_cols(c, true), reuse);
-    Plan Pb;
     cc_status st;
     if (mixed) {
-        Pb = make_plan(c, std::vector<int32_t>{-1}, false);
-        HIPCHK(c, hipEventRecord(c->ev, c->s0));
-        HIPCHK(c, hipStreamWaitEvent(c->s1, c->ev, 0));
-        st = columns(c, Pb, base_buffers(c), d_x, 0.0, false, c->s1);
+        st = mixed_jacobian(c, d_x, nullptr, nullptr);
         if (st != CC_OK) return st;
-    }
-    st = columns(c, Pm, main_buffers(c), d_x, mixed ? c->p.EpsNewton32 : c->p.EpsNewton, mixed, c->s0);
-    if (st != CC_OK) return st;
-    const double eps = mixed ? c->p.EpsNewton32 : c->p.EpsNewton;
-    {
-        const int ncols = (int)Pm.cols.size();
-        HIPCHK(c, up(Pm.cols, c->cols, c->s0));
-        hipLaunchKernelGGL(ccd::k_fd, dim3((ncols + 127) / 128, n), dim3(128), 0, c->s0, c->main.F, c->cols, ncols, 0, n,
-                           eps, c->J);
-        HIPCHK(c, hipGetLastError());
-    }
-    if (mixed) {
-        HIPCHK(c, hipEventRecord(c->ev, c->s1));
-        HIPCHK(c, hipStreamWaitEvent(c->s0, c->ev, 0));
         HIPCHK(c, hipMemcpyAsync(d_F0, c->base.F, sizeof(double) * n, hipMemcpyDeviceToDevice, c->s0));
     } else {
+        Plan Pm = make_plan(c, jacobian_cols(c, true), reuse);
+        st = columns(c, Pm, main_buffers(c), d_x, c->p.EpsNewton, false, c->s0);
+        if (st != CC_OK) return st;
+        const int ncols = (int)Pm.cols.size();
+        HIPCHK(c, up(Pm.cols, c->cols, c->s0));
+        hipLaunchKernelGGL(ccd::k_fd, dim3((ncols + 127) / 128, n), dim3(128), 0, c->s0, c->main.F, c->cols, ncols, 0,
+                           n, c->p.EpsNewton, c->J);
+        HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipMemcpyAsync(d_F0, c->main.F, sizeof(double) * n, hipMemcpyDeviceToDevice, c->s0));
+        if (sync_status) {
+            st = collect_status(c, Pm, main_buffers(c), c->s0, nullptr, true);
+            if (st != CC_OK) return st;
+        }
     }
     hipLaunchKernelGGL(ccd::k_negate_copy, dim3((n + 255) / 256), dim3(256), 0, c->s0, d_F0, c->d, n);
     HIPCHK(c, hipGetLastError());
-    if (sync_status) {
-        if (mixed) {
-            st = collect_status(c, Pb, base_buffers(c), c->s1, nullptr, true);
-            if (st != CC_OK) return st;
-        }
-        st = collect_status(c, Pm, main_buffers(c), c->s0, nullptr, true);
-        if (st != CC_OK) return st;
-    }
     return solve_device(c, n, c->J, c->d, d_dx, nullptr, c->s0);
 }
 
@@ -503,7 +632,24 @@ cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
     ALLOC(c->info, 1);
     ALLOC(c->cols, c->maxcols);
     if (e == hipSuccess) e = alloc_columns(c, c->main, c->maxcols, c->maxchains);
-    if (e == hipSuccess) e = alloc_columns(c, c->base, 1, 2);
+    if (e == hipSuccess) e = alloc_columns(c, c->base, 2, 4);
+    ALLOC(c->rec, (size_t)c->X * (size_t)(2 * (5 * (N / 2) + 2 * N + 4)));   // Rec<N>::STEP floats per step
+    ALLOC(c->rec_base_b, 4);
+    ALLOC(c->rec_base_t, n);
+    ALLOC(c->dY, (size_t)n * M);
+    ALLOC(c->t_start, n);
+    ALLOC(c->t_nsteps, n);
+    ALLOC(c->t_dir, n);
+    ALLOC(c->t_status, n);
+    ALLOC(c->t_fail, n);
+    ALLOC(c->t_cols, n);
+    ALLOC(c->t_lcols, n);
+    ALLOC(c->t_rcols, n);
+    ALLOC(c->t_lch, n);
+    ALLOC(c->t_rch, n);
+    ALLOC(c->t_lidx, n);
+    ALLOC(c->t_ridx, n);
+    ALLOC(c->t_its, n);
 #undef ALLOC
     if (bad(e, "hipMalloc")) return CC_DEVICE;
     // x-grid, NewtonSolver.cpp:477-519
@@ -633,25 +779,28 @@ cc_status cc_jacobian(cc_ctx *c, const double *x, const double *F0, double *J, i
     const bool reuse = c->p.reuse_sides != 0;
     c->counters[0] = c->counters[1] = c->counters[2] = 0;
     HIPCHK(c, hipMemcpyAsync(c->x, x, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
-    // the base column is part of the launch when it is reused or when the
-    // difference quotient needs the FP32 base; otherwise F0 is the base row
-    const bool with_base = reuse || mixed;
-    Plan P = make_plan(c, jacobian_cols(c, with_base), reuse);
-    const double eps = mixed ? c->p.EpsNewton32 : c->p.EpsNewton;
-    cc_status st = columns(c, P, main_buffers(c), c->x, eps, mixed, c->s0);
-    if (st != CC_OK) return st;
-    const int ncols = (int)P.cols.size();
-    int base_row = 0;
-    if (!with_base || !mixed) {
-        // FP64: difference against the caller's F0 (NewtonSolver.cpp:684), stored after the columns
+    cc_status st;
+    if (mixed) {
+        std::vector<int32_t> tst, tcols;
+        st = mixed_jacobian(c, c->x, &tst, &tcols);
+        if (col_status) {
+            for (int i = 0; i < n; ++i) col_status[i] = 0;
+            for (size_t b = 0; b < tcols.size(); ++b) col_status[tcols[b]] = tst[b] ? CC_IRK_NOCONV : CC_OK;
+        }
+    } else {
+        // the base column is in the launch when its sides are reused; the
+        // difference quotient always uses the caller's F0 (NewtonSolver.cpp:684)
+        Plan P = make_plan(c, jacobian_cols(c, reuse), reuse);
+        st = columns(c, P, main_buffers(c), c->x, c->p.EpsNewton, false, c->s0);
+        if (st != CC_OK) return st;
+        const int ncols = (int)P.cols.size();
         HIPCHK(c, hipMemcpyAsync(c->main.F + (size_t)ncols * n, F0, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
-        base_row = ncols;
+        HIPCHK(c, up(P.cols, c->cols, c->s0));
+        hipLaunchKernelGGL(ccd::k_fd, dim3((ncols + 127) / 128, n), dim3(128), 0, c->s0, c->main.F, c->cols, ncols,
+                           ncols, n, c->p.EpsNewton, c->J);
+        HIPCHK(c, hipGetLastError());
+        st = collect_status(c, P, main_buffers(c), c->s0, col_status, true);
     }
-    HIPCHK(c, up(P.cols, c->cols, c->s0));
-    hipLaunchKernelGGL(ccd::k_fd, dim3((ncols + 127) / 128, n), dim3(128), 0, c->s0, c->main.F, c->cols, ncols, base_row,
-                       n, eps, c->J);
-    HIPCHK(c, hipGetLastError());
-    st = collect_status(c, P, main_buffers(c), c->s0, col_status, true);
     HIPCHK(c, hipMemcpyAsync(J, c->J, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));
     return st;

@@ -54,6 +54,22 @@ struct ChainParams {
     double Dim;
     double precision;
     double a00, a01, a10, a11, b0, b1, c0, c1;   // Butcher tableau
+    float *rec;                    // base-trajectory record (tangent-linear columns), may be null
+    const int64_t *rec_base;       // per chain: first record slot, < 0 = do not record
+};
+
+// Base-trajectory record of one IRK step (FP32): for each stage s, the
+// fields of the last RHS evaluation of the step -- U, V, F, dU/dtau, dV/dtau
+// on j < N/2 (the other half follows by symmetry), r = 1/IA2 and
+// E = e^{F_int} on all N points, and lambda = mean(Coeff1).  A
+// tangent-linear chain linearises the RHS around these values.
+template <int N>
+struct Rec {
+    static constexpr int M = N / 2;
+    static constexpr int U = 0, V = M, F = 2 * M, DU = 3 * M, DV = 4 * M, R = 5 * M, E = 5 * M + N,
+                         LAM = 5 * M + 2 * N;
+    static constexpr int STAGE = 5 * M + 2 * N + 4;
+    static constexpr int STEP = 2 * STAGE;
 };
 
 template <typename R>
@@ -68,9 +84,11 @@ struct RhsConst {
 template <int N, int T, typename R>
 __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> (&f)[2][N / T / 2],
                                          const R (&xs)[2], const RhsConst<R> &K, const R *__restrict__ invm,
-                                         cx<R> *xb, R *bc, const cx<R> *__restrict__ tw, int t)
+                                         cx<R> *xb, R *bc, const cx<R> *__restrict__ tw, int t,
+                                         float *rec = nullptr)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
+    using RC = Rec<N>;
     const R half = (R)0.5;
     const R invN = (R)1 / (R)N;
     // Everything derived from the thread index is recomputed per call: held
@@ -122,6 +140,14 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
             const cx<R> a2 = z[1][h], b2 = z[1][h + H];
             const R U = half * (a.x - b.x), V = half * (a.y - b.y), F = half * (a.y + b.y);
             const R dU = half * (a2.x - b2.x), dV = half * (a2.y - b2.y);
+            if (rec) {
+                float *rs = rec + s * RC::STAGE + t + T * h;
+                rs[RC::U] = (float)U;
+                rs[RC::V] = (float)V;
+                rs[RC::F] = (float)F;
+                rs[RC::DU] = (float)dU;
+                rs[RC::DV] = (float)dV;
+            }
             const R C1 = -(((X + F) * U * U + (X - F) * V * V) * K.c3 / ((R)8 * X) + K.dm3);
             if (s == 0) c[h].x = C1; else c[h].y = C1;
             const R two_x = (R)2 * X;
@@ -173,6 +199,10 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     for (int p = 0; p < P; ++p) cm[p] = xb[(N - (t + T * p)) & (N - 1)];
     __syncthreads();
     const R lam0 = bc[2], lam1 = bc[3];
+    if (rec && t == 0) {
+        rec[RC::LAM] = (float)lam0;
+        rec[RC::STAGE + RC::LAM] = (float)lam1;
+    }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         const int k = t + T * p;
@@ -214,6 +244,11 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const R r = E[s][p] / (s ? c[p].y : c[p].x);
+            if (rec) {
+                float *rs = rec + s * RC::STAGE + t + T * p;
+                rs[RC::R] = (float)r;
+                rs[RC::E] = (float)E[s][p];
+            }
             const R cf = CF[s][ph];
             yp[s][p] = {sg * (AU[s][ph] - BU[s][ph] * r), sg * (AV[s][ph] - BV[s][ph] * r) + cf * (r - (R)1)};
         }
@@ -290,9 +325,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         __syncthreads();
         bool conv = false;
         int its = 0;
+        float *rec = nullptr;
+        if (a.rec && a.rec_base[ch] >= 0) rec = a.rec + (size_t)(a.rec_base[ch] + step) * Rec<N>::STEP;
         for (its = 0; its < a.maxIts; ++its) {
             cx<R> f[2][H];
-            rhs_pair<N, T, R>(yk, f, xs, K, a.invm, xb, bc, a.tw, t);
+            rhs_pair<N, T, R>(yk, f, xs, K, a.invm, xb, bc, a.tw, t, rec);
             R loc = (R)0;
 #pragma unroll
             for (int h = 0; h < H; ++h) {

@@ -233,3 +233,33 @@ def test_mixed_solve_refines_to_fp64():
     x, info, _, st = ctx.solve(A, b)
     assert st == 0 and info == 0
     assert rel(x, ref) < 1e-12
+
+
+def test_mixed_tangent_jacobian_matches_oracle_fd(c1_reference):
+    """CC_MIXED Jacobian: FP32 tangent-linear columns + FP64 FD Delta column vs
+    the oracle's FP64 forward differences (eps = 1e-8).  The FD reference has
+    ~3e-8 absolute noise per entry, so columns are compared relative to
+    max(|column|, 1e-4)."""
+    s, F0, J_ref, _, _ = c1_reference
+    N = s["shape"].Ntau
+    ctx = ctx_for(s, precision=cc.CC_MIXED)
+    J, cs, st = ctx.jacobian(s["x"], F0)
+    assert st == 0 and not cs.any()
+    dslot = N // 2 + 2
+    assert rel(J[:, dslot], J_ref[:, dslot]) < 1e-5          # the FP64 FD column
+    cn = np.maximum(np.linalg.norm(J_ref, axis=0), 1e-4)
+    err = np.linalg.norm(J - J_ref, axis=0) / cn
+    print("tangent column error: max", err.max(), "median", np.median(err))
+    assert err.max() < 2e-3
+    assert rel(J, J_ref) < 1e-4
+
+
+def test_mixed_newton_step_solves_fp64_refined_system(c1_reference):
+    """The mixed step solves its own Jacobian system to FP64 accuracy (FP32 LU +
+    FP64 refinement) and F0 is the FP64 residual."""
+    s, F0_ref, _, _, _ = c1_reference
+    ctx = ctx_for(s, precision=cc.CC_MIXED)
+    F0, dx = ctx.newton_step(s["x"])
+    assert np.max(np.abs(F0 - F0_ref)) < 1e-9
+    J, _, _ = ctx.jacobian(s["x"], F0)
+    assert np.linalg.norm(J @ dx + F0) / np.linalg.norm(F0) < 1e-8
