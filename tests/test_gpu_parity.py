@@ -238,8 +238,9 @@ def test_mixed_solve_refines_to_fp64():
 def test_mixed_tangent_jacobian_matches_oracle_fd(c1_reference):
     """CC_MIXED Jacobian: FP32 tangent-linear columns + FP64 FD Delta column vs
     the oracle's FP64 forward differences (eps = 1e-8).  The FD reference has
-    ~3e-8 absolute noise per entry, so columns are compared relative to
-    max(|column|, 1e-4)."""
+    ~3e-8 absolute noise per entry (~3e-7 per column of 96), so columns are
+    compared relative to max(|column|, 1e-4): the weakest psic columns (norms
+    ~6e-5) carry the FD's own noise at the 1e-3 level."""
     s, F0, J_ref, _, _ = c1_reference
     N = s["shape"].Ntau
     ctx = ctx_for(s, precision=cc.CC_MIXED)
@@ -250,7 +251,8 @@ def test_mixed_tangent_jacobian_matches_oracle_fd(c1_reference):
     cn = np.maximum(np.linalg.norm(J_ref, axis=0), 1e-4)
     err = np.linalg.norm(J - J_ref, axis=0) / cn
     print("tangent column error: max", err.max(), "median", np.median(err))
-    assert err.max() < 2e-3
+    assert np.median(err) < 1e-4
+    assert err.max() < 1e-2
     assert rel(J, J_ref) < 1e-4
 
 
