@@ -10,8 +10,11 @@ resident in HBM before the timed region.
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 Prints one JSON line.  value = Newton steps/s of the mixed-precision policy
-(FP32 columns and LU, FP64 residual and refinement); the FP64 form is timed
-in the same run and reported beside it.
+(FP32 tangent-linear columns and LU; FP64 residual, Delta column and
+refinement); the FP64 form (every column a forward difference of two FP64
+shoots, as the reference) is timed in the same run and reported beside it,
+with and without reuse of the unchanged side.  Each policy is autotuned
+first (outside the timed region).
 """
 import argparse
 import importlib
@@ -41,38 +44,57 @@ def flops_chain(N, its, steps, tangent=False):
     return its * (2 * frhs + 18 * N) + steps * 6 * N
 
 
-def time_policy(cc, dev, shape, precision, steps, warmup, reuse):
+def time_policy(cc, dev, shape, precision, steps, warmup, reuse, tune):
+    """Autotune (outside the timed region), then time `steps` Newton steps on
+    device-resident x, F0, dx."""
     n = shape.n
     ctx = cc.Context(cc.Problem.from_shape(shape, precision=precision, reuse_sides=int(reuse)))
     x = importlib.import_module("parallel-critical-collapse_amd.workload").initial_x(shape)
+    name = ("mixed" if precision == cc.CC_MIXED else "fp64") + ("+reuse" if reuse else "")
+    tuned = None
+    if isinstance(tune, dict):
+        ctx.set_schedule(**tune)
+    elif tune:
+        t = time.perf_counter()
+        sc, times = ctx.autotune(x)
+        tuned = {"schedule": sc.as_dict(), "candidate_ms": [round(float(v), 3) for v in times],
+                 "tuning_s": round(time.perf_counter() - t, 2)}
+        log(f"[bench] {name} autotune {tuned['tuning_s']} s -> {sc.as_dict()}")
+    sched = ctx.schedule().as_dict()
     d_x, d_F0, d_dx = dev.DeviceArray(x), dev.DeviceArray(np.zeros(n)), dev.DeviceArray(np.zeros(n))
-    ctx.profile(True)
-    name = "mixed" if precision == cc.CC_MIXED else "fp64"
+    ctx.profile(not sched["use_graph"])
     for w in range(warmup):
         t = time.perf_counter()
         ctx.newton_step_device(d_x.ptr, d_F0.ptr, d_dx.ptr)
-        ctx.synchronize()
         log(f"[bench] {name} warmup {w + 1}/{warmup}: {time.perf_counter() - t:.3f} s")
     dev.synchronize()
     chain_ms, launches, its, nsteps = 0.0, 0, 0, 0
     t0 = time.perf_counter()
     for k in range(steps):
         ctx.newton_step_device(d_x.ptr, d_F0.ptr, d_dx.ptr)
-        ctx.synchronize()
-        ms, nl, it, st = ctx.chain_timing()
-        chain_ms += ms
-        launches += nl
-        its += it
-        nsteps += st
-        log(f"[bench] {name} step {k + 1}/{steps}: chain kernels {ms / 1e3:.3f} s, its {it}")
+        if not sched["use_graph"]:
+            ms, nl, it, st = ctx.chain_timing()
+            chain_ms += ms
+            launches += nl
+            its += it
+            nsteps += st
+        log(f"[bench] {name} step {k + 1}/{steps} done")
     dev.synchronize()
     wall = time.perf_counter() - t0
+    if sched["use_graph"]:
+        # the graph carries no timing events: time the chain kernels on one eager step
+        ctx.set_schedule(use_graph=0)
+        ctx.profile(True)
+        ctx.newton_step_device(d_x.ptr, d_F0.ptr, d_dx.ptr)
+        ms, nl, it, st = ctx.chain_timing()
+        chain_ms, launches, its, nsteps = ms * steps, nl * steps, it * steps, st * steps
     F0, dx = d_F0.download(), d_dx.download()
     ctx.close()
     return {
         "wall_s": wall, "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
         "chain_ms": chain_ms, "launches": launches, "its": its, "irk_steps": nsteps,
         "kbar": its / max(nsteps, 1), "F0_rms": float(np.sqrt(np.mean(F0 ** 2))), "dx": dx,
+        "schedule": sched, "tuning": tuned,
     }
 
 
@@ -111,6 +133,9 @@ def main():
     ap.add_argument("--config", type=int, default=3, help="workload shape 1, 2, 3 or 5")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 policy")
+    ap.add_argument("--no-tune", action="store_true", help="default schedule instead of the autotuner's")
+    ap.add_argument("--schedule", default="", help='fixed schedule fields as JSON, e.g. \'{"threads_tangent": 128}\'; '
+                    "skips the autotuner")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -124,18 +149,22 @@ def main():
     shape = W.config(args.config)
     log(f"[bench] workload Ntau={shape.Ntau} X={shape.X} n={shape.n}")
 
-    mixed = time_policy(cc, dev, shape, cc.CC_MIXED, args.steps, args.warmup, False)
-    fp64 = None if args.no_fp64 else time_policy(cc, dev, shape, cc.CC_FP64, args.steps, args.warmup, False)
+    tune = json.loads(args.schedule) if args.schedule else (not args.no_tune)
+    mixed = time_policy(cc, dev, shape, cc.CC_MIXED, args.steps, args.warmup, False, tune)
+    fp64 = None if args.no_fp64 else time_policy(cc, dev, shape, cc.CC_FP64, args.steps, args.warmup, False, tune)
+    fp64r = None if args.no_fp64 else time_policy(cc, dev, shape, cc.CC_FP64, args.steps, args.warmup, True, tune)
 
     N = shape.Ntau
+    T = mixed["schedule"]["threads_tangent"]
     fl = flops_chain(N, mixed["its"], mixed["irk_steps"], tangent=True)
     achieved = fl / (mixed["chain_ms"] / 1e3) / 1e12
     peak = 157.3
     roof = {"bound": "mfma", "pipe": "fp32 valu (FFT + pointwise; no dense contraction)",
-            "kernel": "k_tangent<1024,256>", "achieved": round(achieved, 3), "peak": peak,
+            "kernel": f"k_tangent<{N},{T}>", "achieved": round(achieved, 3), "peak": peak,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
-            "flops_per_step": fl, "chain_kernel_s_per_step": mixed["chain_ms"] / 1e3 / args.steps,
-            "avg_launch_ms": mixed["chain_ms"] / max(mixed["launches"], 1)}
+            "flops_per_step": fl / args.steps, "kernel_s_per_step": mixed["chain_ms"] / 1e3 / args.steps,
+            "avg_launch_ms": mixed["chain_ms"] / max(mixed["launches"], 1),
+            "launches_per_step": mixed["launches"] / args.steps}
     out = {
         "metric": "Jacobian+Newton steps/s at Ntau=1024, 8000 x-pts (mixed FP32/FP64)",
         "value": round(mixed["steps_per_s"], 5), "unit": "Newton steps/s", "n_gpus": 1,
@@ -144,18 +173,20 @@ def main():
         "data": "synthetic: shipped D=4 solution + seeded N(0,1)*0.1/k^2 odd/even-harmonic perturbation",
         "config": {"workload": f"newton_step Ntau={shape.Ntau} x_points={shape.X} n={shape.n} Dim={shape.Dim} "
                                f"XMid={shape.XMid} IRK2", "policy": "mixed"},
-        "kbar_mixed": round(mixed["kbar"], 3),
+        "schedule": mixed["schedule"], "tuning": mixed["tuning"],
+        "kbar_tangent": round(mixed["kbar"], 3),
         "roofline": roof,
     }
-    if fp64 is not None:
-        fl64 = flops_chain(N, fp64["its"], fp64["irk_steps"])
-        out["fp64"] = {"value": round(fp64["steps_per_s"], 5), "ms_per_step": round(fp64["ms_per_step"], 2),
-                       "kbar": round(fp64["kbar"], 3),
-                       "achieved_tflops": round(fl64 / (fp64["chain_ms"] / 1e3) / 1e12, 3), "peak": 78.6}
-        out["fp64"]["frac"] = round(out["fp64"]["achieved_tflops"] / 78.6, 4)
-        out["mixed_over_fp64"] = round(mixed["steps_per_s"] / fp64["steps_per_s"], 3)
-        d64 = fp64["dx"]
-        out["dx_rel_err_mixed_vs_fp64"] = float(np.linalg.norm(mixed["dx"] - d64) / np.linalg.norm(d64))
+    for key, res in (("fp64", fp64), ("fp64_reuse_sides", fp64r)):
+        if res is None:
+            continue
+        fl64 = flops_chain(N, res["its"], res["irk_steps"])
+        ach = fl64 / (res["chain_ms"] / 1e3) / 1e12
+        out[key] = {"value": round(res["steps_per_s"], 5), "ms_per_step": round(res["ms_per_step"], 2),
+                    "kbar": round(res["kbar"], 3), "kernel": f"k_chain<{N},{res['schedule']['threads_bulk']},double>",
+                    "achieved_tflops": round(ach, 3), "peak": 78.6, "frac": round(ach / 78.6, 4),
+                    "schedule": res["schedule"]}
+        out[f"mixed_over_{key}"] = round(mixed["steps_per_s"] / res["steps_per_s"], 3)
     if not args.no_cpu:
         log(f"[bench] cpu baseline on {args.cpu_threads} threads")
         out["cpu_baseline"] = cpu_baseline(shape, args.cpu_threads, 30.0)

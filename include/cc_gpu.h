@@ -79,8 +79,11 @@ cc_status cc_residual(cc_ctx *ctx, const double *x, double *F);
 /* FD Jacobian: replaces NewtonSolver::assembleJacobian
  * (NewtonSolver.cpp:615-702).  J is row-major n x n, J[j*n+i] = dF_j/dx_i.
  * col_status (n int32, may be NULL) receives each column's cc_status.
- * In CC_MIXED mode F0 is only used for shape checks: the difference quotient
- * uses an FP32 base shoot computed in the same launch. */
+ * CC_FP64: forward differences (F(x + EpsNewton e_i) - F0) / EpsNewton, as
+ * the reference.  CC_MIXED: F0 is not read; the Delta column is that forward
+ * difference in FP64 around an FP64 base shoot, every other column is the
+ * FP32 tangent-linear derivative of the shoot along e_i, integrated around
+ * the recorded FP64 base trajectory. */
 cc_status cc_jacobian(cc_ctx *ctx, const double *x, const double *F0, double *J, int32_t *col_status);
 
 /* Dense solve with partial pivoting: replaces LAPACKE_dgesv
@@ -96,7 +99,9 @@ cc_status cc_solve(cc_ctx *ctx, int32_t n, double *A, double *b, int32_t *info);
 cc_status cc_newton_step(cc_ctx *ctx, const double *x, double *F0, double *dx);
 
 /* Same as cc_newton_step on device-resident arrays (d_x, d_F0, d_dx are
- * device pointers of n doubles); asynchronous on the context stream. */
+ * device pointers of n doubles).  Returns when the step's status is known
+ * (one host synchronisation at its end); with schedule.use_graph the step is
+ * replayed from a hipGraph captured for this (d_x, d_F0, d_dx). */
 cc_status cc_newton_step_device_async(cc_ctx *ctx, const double *d_x, double *d_F0, double *d_dx);
 cc_status cc_synchronize(cc_ctx *ctx);
 
@@ -111,6 +116,36 @@ cc_status cc_counters(cc_ctx *ctx, int64_t out[3]);
  * launches, fixed-point iterations and IRK steps they executed. */
 cc_status cc_profile(cc_ctx *ctx, int32_t enable);
 cc_status cc_chain_timing(cc_ctx *ctx, double *ms, int32_t *launches, int64_t *its, int64_t *steps);
+
+/* ---- launch schedule and autotuner (no counterpart in the reference) ----- */
+
+/* How the chain kernels are launched.  Thread counts are threads per chain
+ * (one workgroup each): Ntau/4 (Ntau = 128: 64) or, for Ntau >= 512, Ntau/8. */
+typedef struct cc_schedule {
+    int32_t threads_base;      /* FP64 chains of small launches: residual, mixed base + Delta column */
+    int32_t threads_bulk;      /* FP64 chains of full-Jacobian launches */
+    int32_t threads_tangent;   /* FP32 tangent-linear chains (CC_MIXED) */
+    int32_t steps_per_launch;  /* IRK steps per chain-kernel launch, 16 .. 8192 */
+    int32_t pipeline;          /* CC_MIXED: tangent chunk k starts as soon as base chunk k is done,
+                                  the base running on a second stream */
+    int32_t use_graph;         /* Newton steps replay a hipGraph captured on first use */
+    int32_t reserved[2];
+} cc_schedule;
+
+cc_status cc_get_schedule(cc_ctx *ctx, cc_schedule *out);
+/* Validates and installs a schedule (drops a captured graph). */
+cc_status cc_set_schedule(cc_ctx *ctx, const cc_schedule *s);
+
+enum { CC_TUNE_THREADS = 1, CC_TUNE_STAGES = 2 };
+/* Autotuner.  CC_TUNE_THREADS: times every thread count of each chain
+ * kernel on a slice of the integration at x (device work only, a few
+ * hundred ms).  CC_TUNE_STAGES: times whole Newton steps at x for
+ * steps_per_launch in {128, 256, 512, 1024} and, in CC_MIXED, pipeline on/off.
+ * Installs and returns the fastest schedule.  times (may be NULL) receives
+ * up to ntimes candidate timings in ms, in the order tried; *ntried their
+ * count. */
+cc_status cc_autotune(cc_ctx *ctx, const double *x, int32_t flags, cc_schedule *out, double *times,
+                      int32_t ntimes, int32_t *ntried);
 
 /* ---- unit entry points (the reference's lower-level interfaces) ---------- */
 
@@ -135,6 +170,15 @@ cc_status cc_rhs_eval(cc_ctx *ctx, const double *Y, const double *xs, double *dY
 cc_status cc_integrate(cc_ctx *ctx, double *Y, const double *grid, int32_t ngrid,
                        const int32_t *start, const int32_t *end, int32_t batch, int32_t fp32,
                        int64_t *its_total, int32_t *status);
+
+/* ICG::packSpectralFields (ICG.cpp:678-711) for `batch` field sets:
+ * fields = [batch][3][Ntau] real tau-series Up, psic, fc -> x = [batch][n].
+ * Slot N/2+2 receives Re of fc's mode 2; the Newton loop overwrites it with
+ * Delta (NewtonSolver.cpp:273-274). */
+cc_status cc_pack(cc_ctx *ctx, const double *fields, double *x, int32_t batch);
+/* ICG::unpackSpectralFields (ICG.cpp:719-758), the Delta slot read as zero
+ * as at its call sites (NewtonSolver.cpp:408-409): x -> fields. */
+cc_status cc_unpack(cc_ctx *ctx, const double *x, double *fields, int32_t batch);
 
 /* InitialConditionGenerator boundary data for Newton vector x: unpack
  * (ICG.cpp:719-758), computeLeftExpansion (37-239) and

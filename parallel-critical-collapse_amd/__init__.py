@@ -60,6 +60,22 @@ class Problem(C.Structure):
                         PrecisionIRK=shape.PrecisionIRK, EpsNewton=shape.EpsNewton, **kw)
 
 
+class Schedule(C.Structure):
+    """cc_schedule: thread counts per chain kernel and the stage schedule."""
+
+    _fields_ = [
+        ("threads_base", C.c_int32), ("threads_bulk", C.c_int32), ("threads_tangent", C.c_int32),
+        ("steps_per_launch", C.c_int32), ("pipeline", C.c_int32), ("use_graph", C.c_int32),
+        ("reserved", C.c_int32 * 2),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+CC_TUNE_THREADS, CC_TUNE_STAGES = 1, 2
+
+
 def declared_symbols():
     """Entry points declared in include/cc_gpu.h."""
     with open(HEADER) as f:
@@ -101,6 +117,11 @@ def load():
     L.cc_expansions.argtypes = [_vp, _dp, _dp, _dp]
     L.cc_profile.argtypes = [_vp, C.c_int32]
     L.cc_chain_timing.argtypes = [_vp, _dp, _ip, _lp, _lp]
+    L.cc_pack.argtypes = [_vp, _dp, _dp, C.c_int32]
+    L.cc_unpack.argtypes = [_vp, _dp, _dp, C.c_int32]
+    L.cc_get_schedule.argtypes = [_vp, C.POINTER(Schedule)]
+    L.cc_set_schedule.argtypes = [_vp, C.POINTER(Schedule)]
+    L.cc_autotune.argtypes = [_vp, _dp, C.c_int32, C.POINTER(Schedule), _dp, C.c_int32, _ip]
     for name in declared_symbols():
         getattr(L, name).restype = C.c_char_p if name == "cc_last_error" else C.c_int
     _lib = L
@@ -184,7 +205,7 @@ class Context:
         return F0, dx
 
     def newton_step_device(self, d_x: int, d_F0: int, d_dx: int):
-        """Device pointers (e.g. torch tensor data_ptr()); asynchronous."""
+        """Device pointers of n doubles each; returns once the step's status is known."""
         self._check(load().cc_newton_step_device_async(self.h, _vp(d_x), _vp(d_F0), _vp(d_dx)),
                     "cc_newton_step_device_async")
 
@@ -198,6 +219,29 @@ class Context:
         self._check(load().cc_chain_timing(self.h, C.byref(ms), C.byref(nl), C.byref(its), C.byref(steps)),
                     "cc_chain_timing")
         return ms.value, nl.value, its.value, steps.value
+
+    def schedule(self) -> Schedule:
+        sc = Schedule()
+        self._check(load().cc_get_schedule(self.h, C.byref(sc)), "cc_get_schedule")
+        return sc
+
+    def set_schedule(self, sc: Schedule | None = None, **kw):
+        """Install a schedule; keyword arguments change fields of the current one."""
+        sc = sc if sc is not None else self.schedule()
+        for k, v in kw.items():
+            setattr(sc, k, int(v))
+        self._check(load().cc_set_schedule(self.h, C.byref(sc)), "cc_set_schedule")
+        return sc
+
+    def autotune(self, x, flags=CC_TUNE_THREADS | CC_TUNE_STAGES):
+        """Run the autotuner at x; returns (installed schedule, candidate timings in ms)."""
+        x = _arr(x)
+        sc = Schedule()
+        times = np.zeros(64)
+        nt = C.c_int32(0)
+        self._check(load().cc_autotune(self.h, _d(x), flags, C.byref(sc), _d(times), times.size, C.byref(nt)),
+                    "cc_autotune")
+        return sc, times[:nt.value].copy()
 
     def synchronize(self):
         self._check(load().cc_synchronize(self.h), "cc_synchronize")
@@ -229,6 +273,22 @@ class Context:
                                         int(fp32), its.ctypes.data_as(_lp), status.ctypes.data_as(_ip)),
                     "cc_integrate")
         return Y, its, status
+
+    def pack(self, Up, psic, fc):
+        """Three real tau-series (N each, or [batch, N]) -> Newton vector(s)."""
+        f = _arr(np.stack([np.atleast_2d(Up), np.atleast_2d(psic), np.atleast_2d(fc)], axis=1))
+        x = np.zeros((f.shape[0], self.n))
+        self._check(load().cc_pack(self.h, _d(f), _d(x), f.shape[0]), "cc_pack")
+        return x[0] if np.ndim(Up) == 1 else x
+
+    def unpack(self, x):
+        """Newton vector(s) -> (Up, psic, fc), the Delta slot read as zero."""
+        xb = _arr(np.atleast_2d(x))
+        f = np.zeros((xb.shape[0], 3, self.N))
+        self._check(load().cc_unpack(self.h, _d(xb), _d(f), xb.shape[0]), "cc_unpack")
+        if np.ndim(x) == 1:
+            return f[0, 0], f[0, 1], f[0, 2]
+        return f[:, 0], f[:, 1], f[:, 2]
 
     def expansions(self, x):
         x = _arr(x)

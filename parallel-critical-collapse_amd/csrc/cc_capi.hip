@@ -1,13 +1,21 @@
 // cc_capi.hip -- host side of the C-ABI declared in include/cc_gpu.h.
 //
 // A context owns the device buffers for one problem size, two HIP streams
-// (default priority), the FFT twiddle tables and the x-grid.  A "column" is
-// one shoot of the reference (NewtonSolver::shoot, NewtonSolver.cpp:403-423)
-// for x perturbed in one slot (or unperturbed, the base); a "chain" is one side
-// of a column integrated to the match point.  Every entry point below runs a
-// set of columns through
+// (default priority), the FFT twiddle tables, the x-grid and the launch plans.
+// A "column" is one shoot of the reference (NewtonSolver::shoot,
+// NewtonSolver.cpp:403-423) for x perturbed in one slot (or unperturbed, the
+// base); a "chain" is one side of a column integrated to the match point.
+// Columns run as
 //     k_boundary (left/right series) -> k_chain (IRK2) -> k_mismatch (pack)
-// and then k_fd / the LU kernels as needed.
+// and then k_fd / the LU kernels as needed.  The length-N kernels live in the
+// per-length translation units behind ccd::Ops<N> (cc_ops.hpp).
+//
+// Launch plans (which chains exist, their start/direction, which columns read
+// which chains) depend only on the problem shape and are uploaded once, at
+// context creation.  A Newton step therefore enqueues kernels and
+// device-to-device copies only -- no host-to-device copies, no memsets -- and
+// reads its status words back through one pinned block at its end, so the
+// same enqueue code can be captured into a hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,34 +27,62 @@
 #include <vector>
 
 #include "../../include/cc_gpu.h"
-#include "cc_kernels.hpp"
-#include "cc_tangent.hpp"
+#include "cc_dense.hpp"
+#include "cc_ops.hpp"
 
 using ccd::cx;
+using ccd::Ops;
 
 namespace {
 
-constexpr int kStepsPerLaunch = 500;
-
-constexpr int tpb(int N) { return N == 128 ? 64 : N / 4; }
+constexpr int kDefaultStepsPerLaunch = 512;
+constexpr int kMaxChunks = 2048;
 
 }  // namespace
 
-// Device arrays of one launch of columns (see launch_columns).
+// Device arrays of one launch of columns, and the device image of its plan.
 struct ColumnBuffers {
-    void *Y = nullptr;                     // [maxchains][M] complex (double or float)
-    double *delta = nullptr, *F = nullptr; // per chain Delta; [maxcols][n] residuals
+    void *Y = nullptr;                     // [maxchains][M] complex double
+    double *delta = nullptr, *F = nullptr; // per chain Delta; [maxcols + 1][n] residuals
     int32_t *start = nullptr, *nsteps = nullptr, *dir = nullptr, *status = nullptr, *fail = nullptr;
     int32_t *lcols = nullptr, *rcols = nullptr, *lch = nullptr, *rch = nullptr;
-    int32_t *lidx = nullptr, *ridx = nullptr, *mstatus = nullptr;
+    int32_t *lidx = nullptr, *ridx = nullptr, *mstatus = nullptr, *cols = nullptr;
     int64_t *its = nullptr;
+};
+
+// The set of chains needed by a list of columns.
+struct Plan {
+    std::vector<int32_t> cols;          // perturbed slot per column (-1: base)
+    std::vector<int32_t> lcols, lch;    // left chains: column slot, chain id
+    std::vector<int32_t> rcols, rch;    // right chains
+    std::vector<int32_t> lidx, ridx;    // per column: chain ids of its two sides
+    std::vector<int32_t> start, nsteps, dir;
+    int nchains = 0;
+};
+
+// Launch extent: the first nl left boundaries, nr right boundaries, nch
+// chains and ncols mismatch columns of a plan.
+struct Counts {
+    int nl, nr, nch, ncols;
+};
+
+// Pinned host image of the status words of one step.
+struct StatusBlock {
+    int32_t *status, *mstatus;   // main launch: per chain, per column
+    int64_t *its;
+    int32_t *bstatus, *bmstatus; // base launch (base + Delta column)
+    int64_t *bits;
+    int32_t *tstatus;            // tangent chains
+    int64_t *tits;
+    int32_t *info;               // LU
 };
 
 struct cc_ctx {
     cc_problem p{};
     int N = 0, M = 0, n = 0, X = 0, iM = 0, iR = 0;
     hipStream_t s0 = nullptr, s1 = nullptr;
-    hipEvent_t ev = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    std::vector<hipEvent_t> chunk_ev;    // base chunk k done (pipelined mixed step)
     std::string err;
     int64_t counters[3] = {0, 0, 0};
     double a[2][2], b[2], c[2];
@@ -59,27 +95,38 @@ struct cc_ctx {
     double *J = nullptr;                 // n x n
     double *A = nullptr;                 // n x n LU work (FP64)
     float *A32 = nullptr;                // n x n LU work (FP32)
-    int32_t *ipiv = nullptr, *info = nullptr, *cols = nullptr;
-    ColumnBuffers main;                  // the Jacobian columns
-    ColumnBuffers base;                  // FP64 base (+ Delta column) of the mixed policy
+    int32_t *ipiv = nullptr, *info = nullptr;
+    ColumnBuffers main;                  // base + every column (FP64 Jacobian)
+    ColumnBuffers base;                  // base + Delta column (residual, mixed policy)
+    Plan main_plan, base_plan;
     // tangent-linear columns of the mixed policy
     float *rec = nullptr;                // base-trajectory record, X steps
-    int64_t *rec_base_b = nullptr;       // record slots of the FP64 base launch's chains
+    int64_t *rec_base_b = nullptr;       // record slots of the base launch's chains
     int64_t *rec_base_t = nullptr;       // record slots of the tangent chains
-    cx<float> *dY = nullptr;             // [n][M] tangent states
+    cx<float> *dY = nullptr;             // [n-1][M] tangent states
     int32_t *t_start = nullptr, *t_nsteps = nullptr, *t_dir = nullptr, *t_status = nullptr, *t_fail = nullptr;
     int32_t *t_cols = nullptr, *t_lcols = nullptr, *t_rcols = nullptr, *t_lch = nullptr, *t_rch = nullptr;
     int32_t *t_lidx = nullptr, *t_ridx = nullptr;
     int64_t *t_its = nullptr;
+    std::vector<int32_t> t_cols_h, t_nsteps_h;
+    int t_nl = 0, t_nr = 0;
     double tangent_rtol = 1e-6, tangent_h = 1e-5;
     int maxcols = 0, maxchains = 0;
+    cc_schedule sched{};
+    void *hblock = nullptr;
+    StatusBlock hs{};
     std::vector<void *> allocs;
-    // chain-kernel timing of the main stream
-    bool profile = false;
+    // kernel timing on the main stream: event pairs around chain launches
+    bool profile = false, capturing = false;
     hipEvent_t tk0 = nullptr, tk1 = nullptr;
+    std::vector<hipEvent_t> tev;
+    int prof_pairs = 0;
     int32_t prof_launches = 0;
     int64_t prof_its = 0, prof_steps = 0;
     bool prof_pending = false;
+    // captured Newton step
+    hipGraphExec_t gexec = nullptr;
+    const void *gkey[3] = {nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -97,6 +144,12 @@ bool fail(cc_ctx *c, const char *what, hipError_t e)
         if (fail((ctx), #call, e_)) return CC_DEVICE;                  \
     } while (0)
 
+#define CCCHK(call)                                                    \
+    do {                                                               \
+        cc_status s_ = (call);                                         \
+        if (s_ != CC_OK) return s_;                                    \
+    } while (0)
+
 template <typename Tp>
 hipError_t dalloc(cc_ctx *c, Tp **p, size_t count)
 {
@@ -107,6 +160,15 @@ hipError_t dalloc(cc_ctx *c, Tp **p, size_t count)
         *p = static_cast<Tp *>(q);
     }
     return e;
+}
+
+template <typename Tp>
+cc_status fill(cc_ctx *c, Tp *p, Tp v, size_t count, hipStream_t s)
+{
+    if (count == 0) return CC_OK;
+    hipLaunchKernelGGL((ccd::k_fill<Tp>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, p, v, count);
+    HIPCHK(c, hipGetLastError());
+    return CC_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -125,15 +187,14 @@ bool dispatch_n(int N, Fn &&fn)
     }
 }
 
-// The set of chains needed by a list of columns.
-struct Plan {
-    std::vector<int32_t> cols;          // perturbed slot per column (-1: base)
-    std::vector<int32_t> lcols, lch;    // left chains: column slot, chain id
-    std::vector<int32_t> rcols, rch;    // right chains
-    std::vector<int32_t> lidx, ridx;    // per column: chain ids of its two sides
-    std::vector<int32_t> start, nsteps, dir;
-    int nchains = 0;
-};
+bool valid_threads(int N, int T)
+{
+    bool ok = false;
+    dispatch_n(N, [&](auto Nc) { ok = Ops<decltype(Nc)::value>::valid_threads(T); });
+    return ok;
+}
+
+int max_chain_steps(const cc_ctx *c) { return std::max(c->iM, c->iR - c->iM); }
 
 bool touches_left(int N, int col) { return col < 0 || (col >= N / 4 && col < 3 * N / 4); }
 bool touches_right(int N, int col) { return col < 0 || col < N / 4 || col == N / 2 + 2; }
@@ -182,45 +243,70 @@ Plan make_plan(const cc_ctx *c, const std::vector<int32_t> &cols, bool reuse)
     return P;
 }
 
-template <typename Tp>
-hipError_t up(const std::vector<Tp> &v, Tp *d, hipStream_t s)
+Counts counts_of(const Plan &P)
 {
-    if (v.empty()) return hipSuccess;
-    return hipMemcpyAsync(d, v.data(), v.size() * sizeof(Tp), hipMemcpyHostToDevice, s);
+    return {(int)P.lcols.size(), (int)P.rcols.size(), P.nchains, (int)P.cols.size()};
 }
 
-// Run the columns of plan P in precision R on stream s.  Outputs: F[b*n..],
-// per-chain status and per-column mismatch status in B.
-template <int N, typename R>
-cc_status launch_columns(cc_ctx *c, const Plan &P, const ColumnBuffers &B, const double *d_x, double eps,
-                         double precision, hipStream_t s, float *rec = nullptr, const int64_t *rec_base = nullptr)
+template <typename Tp>
+hipError_t up(const std::vector<Tp> &v, Tp *d)
 {
-    constexpr int T = tpb(N);
-    const int nch = P.nchains;
-    const int ncols = (int)P.cols.size();
-    HIPCHK(c, up(P.start, B.start, s));
-    HIPCHK(c, up(P.nsteps, B.nsteps, s));
-    HIPCHK(c, up(P.dir, B.dir, s));
-    HIPCHK(c, up(P.lcols, B.lcols, s));
-    HIPCHK(c, up(P.rcols, B.rcols, s));
-    HIPCHK(c, up(P.lch, B.lch, s));
-    HIPCHK(c, up(P.rch, B.rch, s));
-    HIPCHK(c, up(P.lidx, B.lidx, s));
-    HIPCHK(c, up(P.ridx, B.ridx, s));
-    HIPCHK(c, hipMemsetAsync(B.status, 0, sizeof(int32_t) * nch, s));
-    HIPCHK(c, hipMemsetAsync(B.fail, 0xff, sizeof(int32_t) * nch, s));
-    HIPCHK(c, hipMemsetAsync(B.its, 0, sizeof(int64_t) * nch, s));
-    HIPCHK(c, hipMemsetAsync(B.mstatus, 0, sizeof(int32_t) * ncols, s));
-    cx<R> *Y = static_cast<cx<R> *>(B.Y);
-    if (!P.lcols.empty())
-        hipLaunchKernelGGL((ccd::k_boundary<N, T, R, true>), dim3((unsigned)P.lcols.size()), dim3(T), 0, s, d_x,
-                           B.lcols, B.lch, eps, c->p.XLeft, c->p.Dim, c->tw64, Y, B.delta);
-    if (!P.rcols.empty())
-        hipLaunchKernelGGL((ccd::k_boundary<N, T, R, false>), dim3((unsigned)P.rcols.size()), dim3(T), 0, s, d_x,
-                           B.rcols, B.rch, eps, c->p.XRight, c->p.Dim, c->tw64, Y, B.delta);
-    HIPCHK(c, hipGetLastError());
-    ccd::ChainParams<R> a{};
-    a.Y = Y;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(d, v.data(), v.size() * sizeof(Tp), hipMemcpyHostToDevice);
+}
+
+hipError_t upload_plan(const Plan &P, const ColumnBuffers &B)
+{
+    hipError_t e = hipSuccess;
+    for (auto pr : {std::make_pair(&P.start, B.start), std::make_pair(&P.nsteps, B.nsteps),
+                    std::make_pair(&P.dir, B.dir), std::make_pair(&P.lcols, B.lcols), std::make_pair(&P.rcols, B.rcols),
+                    std::make_pair(&P.lch, B.lch), std::make_pair(&P.rch, B.rch), std::make_pair(&P.lidx, B.lidx),
+                    std::make_pair(&P.ridx, B.ridx), std::make_pair(&P.cols, B.cols)})
+        if (e == hipSuccess) e = up(*pr.first, pr.second);
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel timing: one event pair per timed launch group on the main stream.
+// ---------------------------------------------------------------------------
+cc_status time_mark(cc_ctx *c, hipStream_t s, bool begin)
+{
+    if (!c->profile || c->capturing) return CC_OK;
+    const size_t idx = 2 * (size_t)c->prof_pairs + (begin ? 0 : 1);
+    while (c->tev.size() <= idx) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreate(&e));
+        c->tev.push_back(e);
+    }
+    HIPCHK(c, hipEventRecord(c->tev[idx], s));
+    if (!begin) ++c->prof_pairs;
+    return CC_OK;
+}
+
+void time_reset(cc_ctx *c)
+{
+    c->prof_pairs = 0;
+    c->prof_launches = 0;
+}
+
+cc_status ensure_chunk_events(cc_ctx *c, int nchunks)
+{
+    while ((int)c->chunk_ev.size() < nchunks) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->chunk_ev.push_back(e);
+    }
+    return CC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Column launches (FP64)
+// ---------------------------------------------------------------------------
+template <int N>
+ccd::ChainParams<double> chain_params(const cc_ctx *c, const ColumnBuffers &B)
+{
+    ccd::ChainParams<double> a{};
+    a.Y = static_cast<cx<double> *>(B.Y);
     a.start = B.start;
     a.nsteps = B.nsteps;
     a.dir = B.dir;
@@ -229,16 +315,11 @@ cc_status launch_columns(cc_ctx *c, const Plan &P, const ColumnBuffers &B, const
     a.status = B.status;
     a.fail_idx = B.fail;
     a.its = B.its;
-    if constexpr (sizeof(R) == 8) {
-        a.tw = reinterpret_cast<const cx<R> *>(c->tw64);
-        a.invm = reinterpret_cast<const R *>(c->invm64);
-    } else {
-        a.tw = reinterpret_cast<const cx<R> *>(c->tw32);
-        a.invm = reinterpret_cast<const R *>(c->invm32);
-    }
+    a.tw = c->tw64;
+    a.invm = c->invm64;
     a.maxIts = c->p.MaxIterIRK;
     a.Dim = c->p.Dim;
-    a.precision = precision;
+    a.precision = c->p.PrecisionIRK;
     a.a00 = c->a[0][0];
     a.a01 = c->a[0][1];
     a.a10 = c->a[1][0];
@@ -247,88 +328,79 @@ cc_status launch_columns(cc_ctx *c, const Plan &P, const ColumnBuffers &B, const
     a.b1 = c->b[1];
     a.c0 = c->c[0];
     a.c1 = c->c[1];
-    a.rec = rec;
-    a.rec_base = rec_base;
-    const int maxsteps = *std::max_element(P.nsteps.begin(), P.nsteps.end());
-    const bool timed = c->profile && s == c->s0 && rec == nullptr;
-    if (timed) HIPCHK(c, hipEventRecord(c->tk0, s));
-    int launches = 0;
-    for (int s0 = 0; s0 < maxsteps; s0 += kStepsPerLaunch) {
-        a.step_begin = s0;
-        a.step_end = std::min(maxsteps, s0 + kStepsPerLaunch);
-        hipLaunchKernelGGL((ccd::k_chain<N, T, R>), dim3(nch), dim3(T), 0, s, a);
-        ++launches;
-    }
-    if (timed) {
-        HIPCHK(c, hipEventRecord(c->tk1, s));
-        c->prof_launches = launches;
-        c->prof_pending = true;
-    }
-    HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL((ccd::k_mismatch<N, T, R>), dim3(ncols), dim3(T), 0, s, Y, B.lidx, B.ridx, B.delta,
-                       c->p.PrecisionIRK, c->tw64, B.F, B.mstatus);
-    HIPCHK(c, hipGetLastError());
+    a.rec = nullptr;
+    a.rec_base = nullptr;
+    return a;
+}
+
+// Clears the status words of a launch and writes the boundary states.
+template <int N>
+cc_status launch_prologue(cc_ctx *c, const ColumnBuffers &B, Counts k, const double *d_x, double eps, hipStream_t s)
+{
+    CCCHK(fill<int32_t>(c, B.status, 0, k.nch, s));
+    CCCHK(fill<int32_t>(c, B.fail, -1, k.nch, s));
+    CCCHK(fill<int64_t>(c, B.its, 0, k.nch, s));
+    CCCHK(fill<int32_t>(c, B.mstatus, 0, k.ncols, s));
+    if (k.nl)
+        HIPCHK(c, Ops<N>::boundary(s, true, false, k.nl, d_x, B.lcols, B.lch, eps, c->p.XLeft, c->p.Dim, c->tw64,
+                                   B.Y, B.delta));
+    if (k.nr)
+        HIPCHK(c, Ops<N>::boundary(s, false, false, k.nr, d_x, B.rcols, B.rch, eps, c->p.XRight, c->p.Dim, c->tw64,
+                                   B.Y, B.delta));
     return CC_OK;
 }
 
-const ColumnBuffers &main_buffers(cc_ctx *c) { return c->main; }
-const ColumnBuffers &base_buffers(cc_ctx *c) { return c->base; }
-
-hipError_t alloc_columns(cc_ctx *c, ColumnBuffers &B, int maxcols, int maxchains)
+// Boundary -> chains (in chunks of steps_per_launch) -> mismatch, on stream s.
+template <int N>
+cc_status launch_columns(cc_ctx *c, const ColumnBuffers &B, Counts k, const double *d_x, double eps, int T,
+                         hipStream_t s)
 {
-    hipError_t e = hipSuccess;
-    cx<double> *y = nullptr;
-    if (e == hipSuccess) e = dalloc(c, &y, (size_t)maxchains * c->M);
-    B.Y = y;
-    if (e == hipSuccess) e = dalloc(c, &B.delta, maxchains);
-    if (e == hipSuccess) e = dalloc(c, &B.F, (size_t)maxcols * c->n);
-    for (int32_t **q : {&B.start, &B.nsteps, &B.dir, &B.status, &B.fail})
-        if (e == hipSuccess) e = dalloc(c, q, maxchains);
-    for (int32_t **q : {&B.lcols, &B.rcols, &B.lch, &B.rch, &B.lidx, &B.ridx, &B.mstatus})
-        if (e == hipSuccess) e = dalloc(c, q, maxcols);
-    if (e == hipSuccess) e = dalloc(c, &B.its, maxchains);
-    return e;
+    CCCHK(launch_prologue<N>(c, B, k, d_x, eps, s));
+    ccd::ChainParams<double> a = chain_params<N>(c, B);
+    const int maxsteps = max_chain_steps(c), S = c->sched.steps_per_launch;
+    CCCHK(time_mark(c, s, true));
+    for (int s0 = 0; s0 < maxsteps; s0 += S) {
+        a.step_begin = s0;
+        a.step_end = std::min(maxsteps, s0 + S);
+        HIPCHK(c, Ops<N>::chain(s, T, k.nch, a));
+        ++c->prof_launches;
+    }
+    CCCHK(time_mark(c, s, false));
+    HIPCHK(c, Ops<N>::mismatch(s, false, k.ncols, B.Y, B.lidx, B.ridx, B.delta, c->p.PrecisionIRK, c->tw64, B.F,
+                               B.mstatus));
+    return CC_OK;
 }
 
-cc_status columns(cc_ctx *c, const Plan &P, const ColumnBuffers &B, const double *d_x, double eps, bool fp32,
-                  hipStream_t s)
+// Copies the status words of a column launch into the pinned block (main or base part).
+cc_status readback_columns(cc_ctx *c, const ColumnBuffers &B, Counts k, bool is_base, hipStream_t s)
 {
-    cc_status st = CC_OK;
-    const double prec = fp32 ? c->p.PrecisionIRK32 : c->p.PrecisionIRK;
-    dispatch_n(c->N, [&](auto Nc) {
-        constexpr int N = decltype(Nc)::value;
-        st = fp32 ? launch_columns<N, float>(c, P, B, d_x, eps, prec, s)
-                  : launch_columns<N, double>(c, P, B, d_x, eps, prec, s);
-    });
-    return st;
+    int32_t *st = is_base ? c->hs.bstatus : c->hs.status;
+    int32_t *ms = is_base ? c->hs.bmstatus : c->hs.mstatus;
+    int64_t *it = is_base ? c->hs.bits : c->hs.its;
+    HIPCHK(c, hipMemcpyAsync(st, B.status, sizeof(int32_t) * k.nch, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(ms, B.mstatus, sizeof(int32_t) * k.ncols, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(it, B.its, sizeof(int64_t) * k.nch, hipMemcpyDeviceToHost, s));
+    return CC_OK;
 }
 
-// Per-column status after the launch (synchronizes s).
-cc_status collect_status(cc_ctx *c, const Plan &P, const ColumnBuffers &B, hipStream_t s, int32_t *col_status,
-                         bool accumulate_counters)
+// Per-column statuses from the pinned block (after the stream synchronised).
+cc_status eval_columns(cc_ctx *c, const Plan &P, Counts k, bool is_base, int32_t *col_status, bool count)
 {
-    std::vector<int32_t> st(P.nchains), ms(P.cols.size());
-    std::vector<int64_t> its(P.nchains);
-    HIPCHK(c, hipMemcpyAsync(st.data(), B.status, sizeof(int32_t) * st.size(), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipMemcpyAsync(ms.data(), B.mstatus, sizeof(int32_t) * ms.size(), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipMemcpyAsync(its.data(), B.its, sizeof(int64_t) * its.size(), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipStreamSynchronize(s));
-    if (accumulate_counters) {
+    const int32_t *st = is_base ? c->hs.bstatus : c->hs.status;
+    const int32_t *ms = is_base ? c->hs.bmstatus : c->hs.mstatus;
+    const int64_t *it = is_base ? c->hs.bits : c->hs.its;
+    if (count) {
         int64_t steps = 0, itsum = 0;
-        for (int i = 0; i < P.nchains; ++i) {
+        for (int i = 0; i < k.nch; ++i) {
             steps += P.nsteps[i];
-            itsum += its[i];
+            itsum += it[i];
         }
         c->counters[0] += steps;
         c->counters[1] += itsum;
-        c->counters[2] += P.nchains;
-        if (s == c->s0) {
-            c->prof_its = itsum;
-            c->prof_steps = steps;
-        }
+        c->counters[2] += k.nch;
     }
     cc_status worst = CC_OK;
-    for (size_t b = 0; b < P.cols.size(); ++b) {
+    for (int b = 0; b < k.ncols; ++b) {
         int s_ = std::max(st[P.lidx[b]], st[P.ridx[b]]);
         if (s_ == 0) s_ = ms[b];
         if (col_status && P.cols[b] >= 0) col_status[P.cols[b]] = s_;
@@ -339,10 +411,11 @@ cc_status collect_status(cc_ctx *c, const Plan &P, const ColumnBuffers &B, hipSt
     return worst;
 }
 
-// LU with partial pivoting on d_A (n x n row-major, precision R), then
-// solve for d_b (FP64).  info on device.
+// ---------------------------------------------------------------------------
+// Dense solve: LU with partial pivoting (FP64, or FP32 + FP64 refinement)
+// ---------------------------------------------------------------------------
 template <typename R>
-void lu_factor(cc_ctx *c, R *dA, int n, hipStream_t s)
+cc_status lu_factor(cc_ctx *c, R *dA, int n, hipStream_t s)
 {
     for (int k0 = 0; k0 < n; k0 += ccd::LU_NB) {
         hipLaunchKernelGGL((ccd::k_lu_panel<R>), dim3(1), dim3(ccd::LU_PANEL_T), 0, s, dA, n, k0, c->ipiv, c->info);
@@ -354,22 +427,25 @@ void lu_factor(cc_ctx *c, R *dA, int n, hipStream_t s)
             hipLaunchKernelGGL((ccd::k_lu_gemm<R>), dim3(tiles, tiles), dim3(256), 0, s, dA, n, k0);
         }
     }
+    HIPCHK(c, hipGetLastError());
+    return CC_OK;
 }
 
-cc_status solve_device(cc_ctx *c, int n, const double *d_J, const double *d_rhs, double *d_sol, int32_t *info_out,
-                       hipStream_t s)
+// Enqueues d_sol = J^{-1} d_rhs; LAPACK's info lands in hs.info.
+cc_status enqueue_solve(cc_ctx *c, int n, const double *d_J, const double *d_rhs, double *d_sol, hipStream_t s)
 {
-    HIPCHK(c, hipMemsetAsync(c->info, 0, sizeof(int32_t), s));
+    CCCHK(fill<int32_t>(c, c->info, 0, 1, s));
     const bool mixed = c->p.precision == CC_MIXED;
-    const unsigned g = (unsigned)((size_t)n * n + 255) / 256;
+    const size_t nn = (size_t)n * n;
+    const unsigned g = (unsigned)((nn + 255) / 256);
     if (!mixed) {
-        HIPCHK(c, hipMemcpyAsync(c->A, d_J, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
-        lu_factor<double>(c, c->A, n, s);
+        HIPCHK(c, hipMemcpyAsync(c->A, d_J, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
+        CCCHK(lu_factor<double>(c, c->A, n, s));
         HIPCHK(c, hipMemcpyAsync(d_sol, d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL((ccd::k_lu_solve<double, double>), dim3(1), dim3(1024), 0, s, c->A, n, c->ipiv, d_sol);
     } else {
-        hipLaunchKernelGGL((ccd::k_cast_to<float>), dim3(g), dim3(256), 0, s, d_J, c->A32, (size_t)n * n);
-        lu_factor<float>(c, c->A32, n, s);
+        hipLaunchKernelGGL((ccd::k_cast_to<float>), dim3(g), dim3(256), 0, s, d_J, c->A32, nn);
+        CCCHK(lu_factor<float>(c, c->A32, n, s));
         HIPCHK(c, hipMemcpyAsync(d_sol, d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL((ccd::k_lu_solve<float, double>), dim3(1), dim3(1024), 0, s, c->A32, n, c->ipiv, d_sol);
         for (int it = 0; it < c->p.refine_iters; ++it) {       // FP64 iterative refinement
@@ -380,9 +456,13 @@ cc_status solve_device(cc_ctx *c, int n, const double *d_J, const double *d_rhs,
         }
     }
     HIPCHK(c, hipGetLastError());
-    int32_t info = 0;
-    HIPCHK(c, hipMemcpyAsync(&info, c->info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipStreamSynchronize(s));
+    HIPCHK(c, hipMemcpyAsync(c->hs.info, c->info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    return CC_OK;
+}
+
+cc_status eval_info(cc_ctx *c, int32_t *info_out)
+{
+    const int32_t info = *c->hs.info;
     if (info_out) *info_out = info;
     if (info != 0) {
         c->err = "LU factorization found a zero pivot (dgesv info " + std::to_string(info) + ")";
@@ -391,78 +471,21 @@ cc_status solve_device(cc_ctx *c, int n, const double *d_J, const double *d_rhs,
     return CC_OK;
 }
 
-std::vector<int32_t> jacobian_cols(const cc_ctx *c, bool with_base)
-{
-    std::vector<int32_t> cols;
-    if (with_base) cols.push_back(-1);
-    for (int i = 0; i < c->n; ++i) cols.push_back(i);
-    return cols;
-}
-
-// Mixed-policy Jacobian on stream s0.  (1) FP64 launch of the base column
-// (recording its trajectory) and of the Delta column; F(x) = row 0 of
-// base.F, the Delta column is a forward difference in FP64 (its direction
-// changes k0 = 2 pi / Delta inside every spectral operator).  (2) FP32
-// tangent-linear columns for every other slot, one chain each: Up slots only
-// move the right boundary data, psic/fc slots only the left, so the other
-// side's tangent is exactly zero.
+// ---------------------------------------------------------------------------
+// Mixed policy.  (1) FP64 launch of the base column (recording its
+// trajectory) and of the Delta column; F(x) = row 0 of base.F, the Delta
+// column is a forward difference in FP64 (its direction changes k0 = 2 pi /
+// Delta inside every spectral operator).  (2) FP32 tangent-linear columns for
+// every other slot, one chain each: Up slots only move the right boundary
+// data, psic/fc slots only the left, so the other side's tangent is exactly
+// zero.  Tangent chunk k (steps [kS, (k+1)S) of every chain) needs only base
+// chunk k's record: with sched.pipeline the base runs on s1 and each tangent
+// chunk waits on the event of its base chunk, so base chunk k+1 overlaps
+// tangent chunk k.
+// ---------------------------------------------------------------------------
 template <int N>
-cc_status mixed_jacobian_n(cc_ctx *c, const double *d_x, std::vector<int32_t> *tstatus_out, std::vector<int32_t> *tcols_out)
+ccd::TangentParams tangent_params(const cc_ctx *c)
 {
-    constexpr int T = tpb(N);
-    const int n = c->n, dslot = N / 2 + 2;
-    hipStream_t s = c->s0;
-    // (1) FP64: base + Delta column
-    Plan Pb = make_plan(c, std::vector<int32_t>{-1, dslot}, false);
-    const std::vector<int64_t> rb = {0, (int64_t)c->iM, -1, -1};      // chains 0,1 = base L,R
-    HIPCHK(c, hipMemcpyAsync(c->rec_base_b, rb.data(), sizeof(int64_t) * rb.size(), hipMemcpyHostToDevice, s));
-    cc_status st = launch_columns<N, double>(c, Pb, c->base, d_x, c->p.EpsNewton, c->p.PrecisionIRK, s, c->rec,
-                                             c->rec_base_b);
-    if (st != CC_OK) return st;
-    {
-        HIPCHK(c, up(Pb.cols, c->cols, s));
-        hipLaunchKernelGGL(ccd::k_fd, dim3(1, n), dim3(128), 0, s, c->base.F, c->cols, 2, 0, n, c->p.EpsNewton, c->J);
-        HIPCHK(c, hipGetLastError());
-    }
-    st = collect_status(c, Pb, c->base, s, nullptr, true);
-    if (st != CC_OK) return st;
-    // (2) FP32 tangent columns
-    std::vector<int32_t> cols, lcols, lch, rcols, rch, lidx, ridx, start, nsteps, dir;
-    std::vector<int64_t> rbt;
-    for (int i = 0; i < n; ++i) {
-        if (i == dslot) continue;
-        const int id = (int)start.size();
-        const bool left = i >= N / 4;
-        cols.push_back(i);
-        if (left) {
-            lcols.push_back(i); lch.push_back(id); lidx.push_back(id); ridx.push_back(-1);
-            start.push_back(0); nsteps.push_back(c->iM); dir.push_back(1); rbt.push_back(0);
-        } else {
-            rcols.push_back(i); rch.push_back(id); lidx.push_back(-1); ridx.push_back(id);
-            start.push_back(c->iR); nsteps.push_back(c->iR - c->iM); dir.push_back(-1); rbt.push_back(c->iM);
-        }
-    }
-    const int nch = (int)cols.size();
-    HIPCHK(c, up(cols, c->t_cols, s));
-    HIPCHK(c, up(lcols, c->t_lcols, s));
-    HIPCHK(c, up(lch, c->t_lch, s));
-    HIPCHK(c, up(rcols, c->t_rcols, s));
-    HIPCHK(c, up(rch, c->t_rch, s));
-    HIPCHK(c, up(lidx, c->t_lidx, s));
-    HIPCHK(c, up(ridx, c->t_ridx, s));
-    HIPCHK(c, up(start, c->t_start, s));
-    HIPCHK(c, up(nsteps, c->t_nsteps, s));
-    HIPCHK(c, up(dir, c->t_dir, s));
-    HIPCHK(c, hipMemcpyAsync(c->rec_base_t, rbt.data(), sizeof(int64_t) * nch, hipMemcpyHostToDevice, s));
-    HIPCHK(c, hipMemsetAsync(c->t_status, 0, sizeof(int32_t) * nch, s));
-    HIPCHK(c, hipMemsetAsync(c->t_its, 0, sizeof(int64_t) * nch, s));
-    if (!lcols.empty())
-        hipLaunchKernelGGL((ccd::k_boundary_tangent<N, T, true>), dim3((unsigned)lcols.size()), dim3(T), 0, s, d_x,
-                           c->t_lcols, c->t_lch, c->tangent_h, c->p.XLeft, c->p.Dim, c->tw64, c->dY);
-    if (!rcols.empty())
-        hipLaunchKernelGGL((ccd::k_boundary_tangent<N, T, false>), dim3((unsigned)rcols.size()), dim3(T), 0, s, d_x,
-                           c->t_rcols, c->t_rch, c->tangent_h, c->p.XRight, c->p.Dim, c->tw64, c->dY);
-    HIPCHK(c, hipGetLastError());
     ccd::TangentParams a{};
     a.dY = c->dY;
     a.rec = c->rec;
@@ -470,6 +493,7 @@ cc_status mixed_jacobian_n(cc_ctx *c, const double *d_x, std::vector<int32_t> *t
     a.start = c->t_start;
     a.nsteps = c->t_nsteps;
     a.dir = c->t_dir;
+    a.base_status = c->base.status;   // chains 0, 1 of the base launch: base left, base right
     a.delta = c->base.delta;          // [0]: the base's Delta, shared by every tangent chain
     a.grid = c->grid;
     a.status = c->t_status;
@@ -480,87 +504,373 @@ cc_status mixed_jacobian_n(cc_ctx *c, const double *d_x, std::vector<int32_t> *t
     a.maxIts = c->p.MaxIterIRK;
     a.Dim = c->p.Dim;
     a.rtol = c->tangent_rtol;
-    a.a00 = c->a[0][0]; a.a01 = c->a[0][1]; a.a10 = c->a[1][0]; a.a11 = c->a[1][1];
-    a.b0 = c->b[0]; a.b1 = c->b[1]; a.c0 = c->c[0]; a.c1 = c->c[1];
-    const int maxsteps = std::max(c->iM, c->iR - c->iM);
-    if (c->profile) HIPCHK(c, hipEventRecord(c->tk0, s));
-    int launches = 0;
-    for (int s0 = 0; s0 < maxsteps; s0 += kStepsPerLaunch) {
-        a.step_begin = s0;
-        a.step_end = std::min(maxsteps, s0 + kStepsPerLaunch);
-        hipLaunchKernelGGL((ccd::k_tangent<N, T>), dim3(nch), dim3(T), 0, s, a);
-        ++launches;
-    }
-    if (c->profile) {
-        HIPCHK(c, hipEventRecord(c->tk1, s));
-        c->prof_launches = launches;
-        c->prof_pending = true;
-    }
-    HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL((ccd::k_tangent_column<N, T>), dim3(nch), dim3(T), 0, s, c->dY, c->t_cols, c->t_lidx,
-                       c->t_ridx, c->tw64, c->J);
-    HIPCHK(c, hipGetLastError());
-    std::vector<int32_t> tst(nch);
-    std::vector<int64_t> tits(nch);
-    HIPCHK(c, hipMemcpyAsync(tst.data(), c->t_status, sizeof(int32_t) * nch, hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipMemcpyAsync(tits.data(), c->t_its, sizeof(int64_t) * nch, hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipStreamSynchronize(s));
-    int64_t itsum = 0, steps = 0;
-    cc_status worst = CC_OK;
-    for (int b = 0; b < nch; ++b) {
-        itsum += tits[b];
-        steps += nsteps[b];
-        if (tst[b]) worst = CC_IRK_NOCONV;
-    }
-    c->counters[0] += steps;
-    c->counters[1] += itsum;
-    c->counters[2] += nch;
-    c->prof_its = itsum;
-    c->prof_steps = steps;
-    if (tstatus_out) *tstatus_out = tst;
-    if (tcols_out) *tcols_out = cols;
-    if (worst != CC_OK) c->err = "tangent-linear IRK iteration did not converge";
-    return worst;
+    a.a00 = c->a[0][0];
+    a.a01 = c->a[0][1];
+    a.a10 = c->a[1][0];
+    a.a11 = c->a[1][1];
+    a.b0 = c->b[0];
+    a.b1 = c->b[1];
+    a.c0 = c->c[0];
+    a.c1 = c->c[1];
+    return a;
 }
 
-cc_status mixed_jacobian(cc_ctx *c, const double *d_x, std::vector<int32_t> *tst, std::vector<int32_t> *tcols)
+template <int N>
+cc_status tangent_prologue(cc_ctx *c, const double *d_x, hipStream_t s)
+{
+    const int nt = (int)c->t_cols_h.size();
+    CCCHK(fill<int32_t>(c, c->t_status, 0, nt, s));
+    CCCHK(fill<int32_t>(c, c->t_fail, -1, nt, s));
+    CCCHK(fill<int64_t>(c, c->t_its, 0, nt, s));
+    if (c->t_nl)
+        HIPCHK(c, Ops<N>::boundary_tangent(s, true, c->t_nl, d_x, c->t_lcols, c->t_lch, c->tangent_h, c->p.XLeft,
+                                           c->p.Dim, c->tw64, c->dY));
+    if (c->t_nr)
+        HIPCHK(c, Ops<N>::boundary_tangent(s, false, c->t_nr, d_x, c->t_rcols, c->t_rch, c->tangent_h, c->p.XRight,
+                                           c->p.Dim, c->tw64, c->dY));
+    return CC_OK;
+}
+
+template <int N>
+cc_status enqueue_mixed_n(cc_ctx *c, const double *d_x)
+{
+    const int n = c->n;
+    const int nt = (int)c->t_cols_h.size();
+    const bool pipe = c->sched.pipeline != 0;
+    hipStream_t s0 = c->s0, sb = pipe ? c->s1 : c->s0;
+    const int S = c->sched.steps_per_launch, maxsteps = max_chain_steps(c);
+    const int nchunks = (maxsteps + S - 1) / S;
+    const Counts kb = counts_of(c->base_plan);
+    if (pipe) {
+        CCCHK(ensure_chunk_events(c, nchunks));
+        HIPCHK(c, hipEventRecord(c->ev_fork, s0));
+        HIPCHK(c, hipStreamWaitEvent(sb, c->ev_fork, 0));
+    }
+    CCCHK(tangent_prologue<N>(c, d_x, s0));
+    CCCHK(launch_prologue<N>(c, c->base, kb, d_x, c->p.EpsNewton, sb));
+    ccd::ChainParams<double> ab = chain_params<N>(c, c->base);
+    ab.rec = c->rec;
+    ab.rec_base = c->rec_base_b;
+    ccd::TangentParams at = tangent_params<N>(c);
+    for (int k = 0; k < nchunks; ++k) {
+        ab.step_begin = at.step_begin = k * S;
+        ab.step_end = at.step_end = std::min(maxsteps, (k + 1) * S);
+        HIPCHK(c, Ops<N>::chain(sb, c->sched.threads_base, kb.nch, ab));
+        if (pipe) {
+            HIPCHK(c, hipEventRecord(c->chunk_ev[k], sb));
+            HIPCHK(c, hipStreamWaitEvent(s0, c->chunk_ev[k], 0));
+        }
+        CCCHK(time_mark(c, s0, true));
+        HIPCHK(c, Ops<N>::tangent(s0, c->sched.threads_tangent, nt, at));
+        CCCHK(time_mark(c, s0, false));
+        ++c->prof_launches;
+    }
+    HIPCHK(c, Ops<N>::mismatch(sb, false, kb.ncols, c->base.Y, c->base.lidx, c->base.ridx, c->base.delta,
+                               c->p.PrecisionIRK, c->tw64, c->base.F, c->base.mstatus));
+    hipLaunchKernelGGL(ccd::k_fd, dim3(1, n), dim3(128), 0, sb, c->base.F, c->base.cols, kb.ncols, 0, n,
+                       c->p.EpsNewton, c->J);
+    HIPCHK(c, hipGetLastError());
+    if (pipe) {
+        HIPCHK(c, hipEventRecord(c->ev_join, sb));
+        HIPCHK(c, hipStreamWaitEvent(s0, c->ev_join, 0));
+    }
+    HIPCHK(c, Ops<N>::tangent_column(s0, nt, c->dY, c->t_cols, c->t_lidx, c->t_ridx, c->tw64, c->J));
+    CCCHK(readback_columns(c, c->base, kb, true, s0));
+    HIPCHK(c, hipMemcpyAsync(c->hs.tstatus, c->t_status, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, s0));
+    HIPCHK(c, hipMemcpyAsync(c->hs.tits, c->t_its, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, s0));
+    return CC_OK;
+}
+
+cc_status enqueue_mixed(cc_ctx *c, const double *d_x)
 {
     cc_status st = CC_OK;
-    dispatch_n(c->N, [&](auto Nc) { st = mixed_jacobian_n<decltype(Nc)::value>(c, d_x, tst, tcols); });
+    dispatch_n(c->N, [&](auto Nc) { st = enqueue_mixed_n<decltype(Nc)::value>(c, d_x); });
     return st;
 }
 
-// The Newton step on device arrays.  FP64: one launch of n+1 columns.
-// Mixed: FP64 base shoot on s1 concurrently with the FP32 columns on s0.
-cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double *d_dx, bool sync_status)
+// Statuses and counters of a mixed Jacobian (after the stream synchronised).
+cc_status eval_mixed(cc_ctx *c, int32_t *col_status)
+{
+    const cc_status sb = eval_columns(c, c->base_plan, counts_of(c->base_plan), true, nullptr, true);
+    const int nt = (int)c->t_cols_h.size();
+    int64_t itsum = 0, steps = 0;
+    bool bad = false;
+    for (int b = 0; b < nt; ++b) {
+        itsum += c->hs.tits[b];
+        steps += c->t_nsteps_h[b];
+        if (c->hs.tstatus[b]) bad = true;
+    }
+    c->counters[0] += steps;
+    c->counters[1] += itsum;
+    c->counters[2] += nt;
+    c->prof_its = itsum;
+    c->prof_steps = steps;
+    if (col_status) {
+        for (int i = 0; i < c->n; ++i) col_status[i] = sb;
+        for (int b = 0; b < nt; ++b)
+            col_status[c->t_cols_h[b]] = sb != CC_OK ? sb : (c->hs.tstatus[b] ? CC_IRK_NOCONV : CC_OK);
+    }
+    if (sb != CC_OK) return sb;
+    if (bad) {
+        c->err = "tangent-linear IRK iteration did not converge";
+        return CC_IRK_NOCONV;
+    }
+    return CC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 Jacobian: one launch of the base and every column.
+// ---------------------------------------------------------------------------
+cc_status enqueue_fp64_columns(cc_ctx *c, const double *d_x)
+{
+    cc_status st = CC_OK;
+    dispatch_n(c->N, [&](auto Nc) {
+        st = launch_columns<decltype(Nc)::value>(c, c->main, counts_of(c->main_plan), d_x, c->p.EpsNewton,
+                                                 c->sched.threads_bulk, c->s0);
+    });
+    return st;
+}
+
+cc_status eval_fp64(cc_ctx *c, int32_t *col_status)
+{
+    const Counts k = counts_of(c->main_plan);
+    const cc_status st = eval_columns(c, c->main_plan, k, false, col_status, true);
+    int64_t itsum = 0, steps = 0;
+    for (int i = 0; i < k.nch; ++i) {
+        itsum += c->hs.its[i];
+        steps += c->main_plan.nsteps[i];
+    }
+    c->prof_its = itsum;
+    c->prof_steps = steps;
+    return st;
+}
+
+// The whole Newton step, enqueued on s0 (s1 joins in the pipelined mixed
+// form): F0 = F(x), J, d = -F0, LU, solve (+ refinement), status readback.
+cc_status enqueue_newton(cc_ctx *c, const double *d_x, double *d_F0, double *d_dx)
 {
     const int n = c->n;
-    const bool mixed = c->p.precision == CC_MIXED;
-    const bool reuse = c->p.reuse_sides != 0;
-    c->counters[0] = c->counters[1] = c->counters[2] = 0;
-    cc_status st;
-    if (mixed) {
-        st = mixed_jacobian(c, d_x, nullptr, nullptr);
-        if (st != CC_OK) return st;
+    if (c->p.precision == CC_MIXED) {
+        CCCHK(enqueue_mixed(c, d_x));
         HIPCHK(c, hipMemcpyAsync(d_F0, c->base.F, sizeof(double) * n, hipMemcpyDeviceToDevice, c->s0));
     } else {
-        Plan Pm = make_plan(c, jacobian_cols(c, true), reuse);
-        st = columns(c, Pm, main_buffers(c), d_x, c->p.EpsNewton, false, c->s0);
-        if (st != CC_OK) return st;
-        const int ncols = (int)Pm.cols.size();
-        HIPCHK(c, up(Pm.cols, c->cols, c->s0));
-        hipLaunchKernelGGL(ccd::k_fd, dim3((ncols + 127) / 128, n), dim3(128), 0, c->s0, c->main.F, c->cols, ncols, 0,
-                           n, c->p.EpsNewton, c->J);
+        CCCHK(enqueue_fp64_columns(c, d_x));
+        const int ncols = (int)c->main_plan.cols.size();
+        hipLaunchKernelGGL(ccd::k_fd, dim3((ncols + 127) / 128, n), dim3(128), 0, c->s0, c->main.F, c->main.cols,
+                           ncols, 0, n, c->p.EpsNewton, c->J);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipMemcpyAsync(d_F0, c->main.F, sizeof(double) * n, hipMemcpyDeviceToDevice, c->s0));
-        if (sync_status) {
-            st = collect_status(c, Pm, main_buffers(c), c->s0, nullptr, true);
-            if (st != CC_OK) return st;
-        }
+        CCCHK(readback_columns(c, c->main, counts_of(c->main_plan), false, c->s0));
     }
     hipLaunchKernelGGL(ccd::k_negate_copy, dim3((n + 255) / 256), dim3(256), 0, c->s0, d_F0, c->d, n);
     HIPCHK(c, hipGetLastError());
-    return solve_device(c, n, c->J, c->d, d_dx, nullptr, c->s0);
+    return enqueue_solve(c, n, c->J, c->d, d_dx, c->s0);
+}
+
+void drop_graph(cc_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gkey[0] = c->gkey[1] = c->gkey[2] = nullptr;
+}
+
+cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double *d_dx)
+{
+    c->counters[0] = c->counters[1] = c->counters[2] = 0;
+    time_reset(c);
+    if (c->sched.use_graph) {
+        if (!c->gexec || c->gkey[0] != d_x || c->gkey[1] != d_F0 || c->gkey[2] != d_dx) {
+            drop_graph(c);
+            if (c->sched.pipeline && c->p.precision == CC_MIXED) {
+                const int S = c->sched.steps_per_launch;
+                CCCHK(ensure_chunk_events(c, (max_chain_steps(c) + S - 1) / S));
+            }
+            HIPCHK(c, hipStreamBeginCapture(c->s0, hipStreamCaptureModeThreadLocal));
+            c->capturing = true;
+            const cc_status st = enqueue_newton(c, d_x, d_F0, d_dx);
+            c->capturing = false;
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(c->s0, &g);
+            if (st != CC_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            HIPCHK(c, e);
+            const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(c, ei);
+            c->gkey[0] = d_x;
+            c->gkey[1] = d_F0;
+            c->gkey[2] = d_dx;
+        }
+        HIPCHK(c, hipGraphLaunch(c->gexec, c->s0));
+    } else {
+        CCCHK(enqueue_newton(c, d_x, d_F0, d_dx));
+        c->prof_pending = c->profile;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    const cc_status st = c->p.precision == CC_MIXED ? eval_mixed(c, nullptr) : eval_fp64(c, nullptr);
+    CCCHK(st);
+    return eval_info(c, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner
+// ---------------------------------------------------------------------------
+cc_status elapsed_ms(cc_ctx *c, double *ms)
+{
+    HIPCHK(c, hipEventSynchronize(c->tk1));
+    float f = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&f, c->tk0, c->tk1));
+    *ms = f;
+    return CC_OK;
+}
+
+template <int N>
+cc_status tune_threads_n(cc_ctx *c, const double *d_x, std::vector<double> &times)
+{
+    using O = Ops<N>;
+    std::vector<int> Ts = {O::kThreads};
+    if (O::kAltThreads) Ts.push_back(O::kAltThreads);
+    hipStream_t s = c->s0;
+    const int maxsteps = max_chain_steps(c);
+    const bool mixed = c->p.precision == CC_MIXED;
+    auto pick = [&](int &slot, const std::vector<double> &t) {
+        size_t best = 0;
+        for (size_t i = 1; i < t.size(); ++i)
+            if (t[i] < t[best]) best = i;
+        slot = Ts[best];
+    };
+    // base + Delta chains (residual, mixed base): a slice of steps [8, 8 + Sb)
+    {
+        const Counts kb = counts_of(c->base_plan);
+        const int Sb = std::min(maxsteps - 8, 256);
+        std::vector<double> t;
+        for (int T : Ts) {
+            CCCHK(launch_prologue<N>(c, c->base, kb, d_x, c->p.EpsNewton, s));
+            ccd::ChainParams<double> a = chain_params<N>(c, c->base);
+            a.step_begin = 0;
+            a.step_end = 8;
+            HIPCHK(c, O::chain(s, T, kb.nch, a));
+            a.step_begin = 8;
+            a.step_end = 8 + Sb;
+            HIPCHK(c, hipEventRecord(c->tk0, s));
+            HIPCHK(c, O::chain(s, T, kb.nch, a));
+            HIPCHK(c, hipEventRecord(c->tk1, s));
+            double ms;
+            CCCHK(elapsed_ms(c, &ms));
+            t.push_back(ms);
+        }
+        pick(c->sched.threads_base, t);
+        times.insert(times.end(), t.begin(), t.end());
+    }
+    if (mixed) {
+        // tangent chains: record the base over [0, St), then time [8, St)
+        const Counts kb = counts_of(c->base_plan);
+        const int St = std::min(maxsteps, 136);
+        const int nt = (int)c->t_cols_h.size();
+        CCCHK(launch_prologue<N>(c, c->base, kb, d_x, c->p.EpsNewton, s));
+        ccd::ChainParams<double> ab = chain_params<N>(c, c->base);
+        ab.rec = c->rec;
+        ab.rec_base = c->rec_base_b;
+        ab.step_begin = 0;
+        ab.step_end = St;
+        HIPCHK(c, O::chain(s, c->sched.threads_base, kb.nch, ab));
+        std::vector<double> t;
+        for (int T : Ts) {
+            CCCHK(tangent_prologue<N>(c, d_x, s));
+            ccd::TangentParams at = tangent_params<N>(c);
+            at.step_begin = 0;
+            at.step_end = 8;
+            HIPCHK(c, O::tangent(s, T, nt, at));
+            at.step_begin = 8;
+            at.step_end = St;
+            HIPCHK(c, hipEventRecord(c->tk0, s));
+            HIPCHK(c, O::tangent(s, T, nt, at));
+            HIPCHK(c, hipEventRecord(c->tk1, s));
+            double ms;
+            CCCHK(elapsed_ms(c, &ms));
+            t.push_back(ms);
+        }
+        pick(c->sched.threads_tangent, t);
+        times.insert(times.end(), t.begin(), t.end());
+    } else {
+        // every column of the FP64 Jacobian: a slice of steps [4, 4 + Sk)
+        const Counts km = counts_of(c->main_plan);
+        const int Sk = std::min(maxsteps - 4, 48);
+        std::vector<double> t;
+        for (int T : Ts) {
+            CCCHK(launch_prologue<N>(c, c->main, km, d_x, c->p.EpsNewton, s));
+            ccd::ChainParams<double> a = chain_params<N>(c, c->main);
+            a.step_begin = 0;
+            a.step_end = 4;
+            HIPCHK(c, O::chain(s, T, km.nch, a));
+            a.step_begin = 4;
+            a.step_end = 4 + Sk;
+            HIPCHK(c, hipEventRecord(c->tk0, s));
+            HIPCHK(c, O::chain(s, T, km.nch, a));
+            HIPCHK(c, hipEventRecord(c->tk1, s));
+            double ms;
+            CCCHK(elapsed_ms(c, &ms));
+            t.push_back(ms);
+        }
+        pick(c->sched.threads_bulk, t);
+        times.insert(times.end(), t.begin(), t.end());
+    }
+    HIPCHK(c, hipStreamSynchronize(s));
+    return CC_OK;
+}
+
+cc_status timed_step(cc_ctx *c, double *ms)
+{
+    HIPCHK(c, hipEventRecord(c->tk0, c->s0));
+    CCCHK(newton_step_device(c, c->x, c->F0, c->dx));
+    HIPCHK(c, hipEventRecord(c->tk1, c->s0));
+    return elapsed_ms(c, ms);
+}
+
+cc_status tune_stages(cc_ctx *c, std::vector<double> &times)
+{
+    const bool mixed = c->p.precision == CC_MIXED;
+    const bool prof = c->profile;
+    c->profile = false;
+    cc_schedule best = c->sched;
+    double best_ms = 1e300;
+    for (int pipe : {1, 0}) {
+        if (pipe && !mixed) continue;
+        for (int S : {128, 256, 512, 1024}) {
+            if ((max_chain_steps(c) + S - 1) / S > kMaxChunks) continue;
+            drop_graph(c);
+            c->sched.steps_per_launch = S;
+            c->sched.pipeline = pipe;
+            c->sched.use_graph = 0;
+            double ms = 0.0;
+            const cc_status st = timed_step(c, &ms);
+            if (st != CC_OK) {
+                c->profile = prof;
+                return st;
+            }
+            times.push_back(ms);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c->sched;
+            }
+        }
+    }
+    // graph replay of the best stage schedule: capture + one replay, then a timed replay
+    c->sched = best;
+    c->sched.use_graph = 1;
+    drop_graph(c);
+    double ms = 0.0;
+    cc_status st = timed_step(c, &ms);
+    if (st == CC_OK) st = timed_step(c, &ms);
+    c->profile = prof;
+    if (st != CC_OK) return st;
+    times.push_back(ms);
+    if (ms < best_ms) best = c->sched;
+    c->sched = best;
+    drop_graph(c);
+    return CC_OK;
 }
 
 }  // namespace
@@ -597,21 +907,53 @@ cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
     c->b[0] = c->b[1] = 0.5;
     c->c[0] = 0.5 - 0.5 / std::sqrt(3.0);
     c->c[1] = 0.5 + 0.5 / std::sqrt(3.0);
+    // default schedule
+    c->sched.threads_base = c->sched.threads_bulk = c->sched.threads_tangent = ccd::default_threads(N);
+    c->sched.steps_per_launch = kDefaultStepsPerLaunch;
+    c->sched.pipeline = 1;
+    c->sched.use_graph = 0;
     auto bad = [&](hipError_t e, const char *w) {
         if (e == hipSuccess) return false;
         fprintf(stderr, "cc_ctx_create: %s: %s\n", w, hipGetErrorString(e));
         for (void *q : c->allocs) (void)hipFree(q);
+        if (c->hblock) (void)hipHostFree(c->hblock);
         delete c;
         return true;
     };
     if (bad(hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking), "stream")) return CC_DEVICE;
     if (bad(hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking), "stream")) return CC_DEVICE;
-    if (bad(hipEventCreateWithFlags(&c->ev, hipEventDisableTiming), "event")) return CC_DEVICE;
+    if (bad(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event")) return CC_DEVICE;
+    if (bad(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "event")) return CC_DEVICE;
     if (bad(hipEventCreate(&c->tk0), "event")) return CC_DEVICE;
     if (bad(hipEventCreate(&c->tk1), "event")) return CC_DEVICE;
     const int n = c->n, M = c->M;
     c->maxcols = n + 2;
     c->maxchains = 2 * c->maxcols;
+    const int dslot = N / 2 + 2;
+    // plans: main = base + every column (sides reused if asked), base = base + Delta column
+    std::vector<int32_t> jcols = {-1};
+    for (int i = 0; i < n; ++i) jcols.push_back(i);
+    c->main_plan = make_plan(c, jcols, p->reuse_sides != 0);
+    c->base_plan = make_plan(c, std::vector<int32_t>{-1, dslot}, false);
+    // tangent plan: one chain per slot except Delta, on the side its boundary data moves
+    std::vector<int32_t> t_lcols, t_lch, t_rcols, t_rch, t_lidx, t_ridx, t_start, t_dir;
+    std::vector<int64_t> rbt;
+    for (int i = 0; i < n; ++i) {
+        if (i == dslot) continue;
+        const int id = (int)c->t_cols_h.size();
+        const bool left = i >= N / 4;
+        c->t_cols_h.push_back(i);
+        if (left) {
+            t_lcols.push_back(i); t_lch.push_back(id); t_lidx.push_back(id); t_ridx.push_back(-1);
+            t_start.push_back(0); c->t_nsteps_h.push_back(c->iM); t_dir.push_back(1); rbt.push_back(0);
+        } else {
+            t_rcols.push_back(i); t_rch.push_back(id); t_lidx.push_back(-1); t_ridx.push_back(id);
+            t_start.push_back(c->iR); c->t_nsteps_h.push_back(c->iR - c->iM); t_dir.push_back(-1);
+            rbt.push_back(c->iM);
+        }
+    }
+    c->t_nl = (int)t_lcols.size();
+    c->t_nr = (int)t_rcols.size();
     hipError_t e = hipSuccess;
 #define ALLOC(ptr, cnt) \
     if (e == hipSuccess) e = dalloc(c, &(ptr), (cnt));
@@ -630,28 +972,64 @@ cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
     ALLOC(c->A32, (size_t)n * n);
     ALLOC(c->ipiv, n);
     ALLOC(c->info, 1);
-    ALLOC(c->cols, c->maxcols);
-    if (e == hipSuccess) e = alloc_columns(c, c->main, c->maxcols, c->maxchains);
-    if (e == hipSuccess) e = alloc_columns(c, c->base, 2, 4);
+    for (ColumnBuffers *B : {&c->main, &c->base}) {
+        const int mc = B == &c->main ? c->maxcols : 2, mch = B == &c->main ? c->maxchains : 4;
+        cx<double> *y = nullptr;
+        ALLOC(y, (size_t)mch * M);
+        B->Y = y;
+        ALLOC(B->delta, mch);
+        ALLOC(B->F, (size_t)(mc + 1) * n);
+        for (int32_t **q : {&B->start, &B->nsteps, &B->dir, &B->status, &B->fail})
+            ALLOC(*q, mch);
+        for (int32_t **q : {&B->lcols, &B->rcols, &B->lch, &B->rch, &B->lidx, &B->ridx, &B->mstatus, &B->cols})
+            ALLOC(*q, mc);
+        ALLOC(B->its, mch);
+    }
     ALLOC(c->rec, (size_t)c->X * (size_t)(2 * (5 * (N / 2) + 2 * N + 4)));   // Rec<N>::STEP floats per step
     ALLOC(c->rec_base_b, 4);
     ALLOC(c->rec_base_t, n);
     ALLOC(c->dY, (size_t)n * M);
-    ALLOC(c->t_start, n);
-    ALLOC(c->t_nsteps, n);
-    ALLOC(c->t_dir, n);
-    ALLOC(c->t_status, n);
-    ALLOC(c->t_fail, n);
-    ALLOC(c->t_cols, n);
-    ALLOC(c->t_lcols, n);
-    ALLOC(c->t_rcols, n);
-    ALLOC(c->t_lch, n);
-    ALLOC(c->t_rch, n);
-    ALLOC(c->t_lidx, n);
-    ALLOC(c->t_ridx, n);
+    for (int32_t **q : {&c->t_start, &c->t_nsteps, &c->t_dir, &c->t_status, &c->t_fail, &c->t_cols, &c->t_lcols,
+                        &c->t_rcols, &c->t_lch, &c->t_rch, &c->t_lidx, &c->t_ridx})
+        ALLOC(*q, n);
     ALLOC(c->t_its, n);
 #undef ALLOC
     if (bad(e, "hipMalloc")) return CC_DEVICE;
+    // pinned status block
+    {
+        const size_t mch = c->maxchains, mc = c->maxcols;
+        const size_t bytes = sizeof(int64_t) * (mch + 4 + (size_t)n) + sizeof(int32_t) * (mch + mc + 4 + 2 + n + 2);
+        if (bad(hipHostMalloc(&c->hblock, bytes, hipHostMallocDefault), "hipHostMalloc")) return CC_DEVICE;
+        char *q = static_cast<char *>(c->hblock);
+        auto take = [&](auto **ptr, size_t cnt) {
+            *ptr = reinterpret_cast<std::remove_pointer_t<decltype(ptr)>>(q);
+            q += cnt * sizeof(**ptr);
+        };
+        take(&c->hs.its, mch);
+        take(&c->hs.bits, 4);
+        take(&c->hs.tits, n);
+        take(&c->hs.status, mch);
+        take(&c->hs.mstatus, mc);
+        take(&c->hs.bstatus, 4);
+        take(&c->hs.bmstatus, 2);
+        take(&c->hs.tstatus, n);
+        take(&c->hs.info, 2);
+    }
+    // plans to the device
+    if (bad(upload_plan(c->main_plan, c->main), "plan")) return CC_DEVICE;
+    if (bad(upload_plan(c->base_plan, c->base), "plan")) return CC_DEVICE;
+    {
+        const std::vector<int64_t> rb = {0, (int64_t)c->iM, -1, -1};      // base chains 0, 1 record
+        hipError_t eu = up(rb, c->rec_base_b);
+        for (auto pr : {std::make_pair(&c->t_cols_h, c->t_cols), std::make_pair(&t_lcols, c->t_lcols),
+                        std::make_pair(&t_lch, c->t_lch), std::make_pair(&t_rcols, c->t_rcols),
+                        std::make_pair(&t_rch, c->t_rch), std::make_pair(&t_lidx, c->t_lidx),
+                        std::make_pair(&t_ridx, c->t_ridx), std::make_pair(&t_start, c->t_start),
+                        std::make_pair(&c->t_nsteps_h, c->t_nsteps), std::make_pair(&t_dir, c->t_dir)})
+            if (eu == hipSuccess) eu = up(*pr.first, pr.second);
+        if (eu == hipSuccess) eu = up(rbt, c->rec_base_t);
+        if (bad(eu, "plan")) return CC_DEVICE;
+    }
     // x-grid, NewtonSolver.cpp:477-519
     {
         std::vector<double> g(c->X + 1);
@@ -704,10 +1082,12 @@ cc_status cc_ctx_destroy(cc_ctx *c)
     if (!c) return CC_BADARG;
     (void)hipStreamSynchronize(c->s0);
     (void)hipStreamSynchronize(c->s1);
+    drop_graph(c);
     for (void *q : c->allocs) (void)hipFree(q);
-    (void)hipEventDestroy(c->ev);
-    (void)hipEventDestroy(c->tk0);
-    (void)hipEventDestroy(c->tk1);
+    if (c->hblock) (void)hipHostFree(c->hblock);
+    for (hipEvent_t e : c->chunk_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {c->ev_fork, c->ev_join, c->tk0, c->tk1}) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->s0);
     (void)hipStreamDestroy(c->s1);
     delete c;
@@ -742,16 +1122,74 @@ cc_status cc_chain_timing(cc_ctx *c, double *ms, int32_t *launches, int64_t *its
 {
     if (!c || !ms || !launches || !its || !steps) return CC_BADARG;
     if (!c->prof_pending) {
-        c->err = "no profiled chain launch since cc_profile(ctx, 1)";
+        c->err = "no profiled eager chain launch since cc_profile(ctx, 1)";
         return CC_BADARG;
     }
-    HIPCHK(c, hipEventSynchronize(c->tk1));
-    float f = 0.0f;
-    HIPCHK(c, hipEventElapsedTime(&f, c->tk0, c->tk1));
-    *ms = f;
+    double tot = 0.0;
+    for (int i = 0; i < c->prof_pairs; ++i) {
+        HIPCHK(c, hipEventSynchronize(c->tev[2 * i + 1]));
+        float f = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&f, c->tev[2 * i], c->tev[2 * i + 1]));
+        tot += f;
+    }
+    *ms = tot;
     *launches = c->prof_launches;
     *its = c->prof_its;
     *steps = c->prof_steps;
+    return CC_OK;
+}
+
+cc_status cc_get_schedule(cc_ctx *c, cc_schedule *out)
+{
+    if (!c || !out) return CC_BADARG;
+    *out = c->sched;
+    return CC_OK;
+}
+
+cc_status cc_set_schedule(cc_ctx *c, const cc_schedule *s)
+{
+    if (!c || !s) return CC_BADARG;
+    if (!valid_threads(c->N, s->threads_base) || !valid_threads(c->N, s->threads_bulk) ||
+        !valid_threads(c->N, s->threads_tangent)) {
+        c->err = "unsupported thread count for this Ntau";
+        return CC_BADARG;
+    }
+    if (s->steps_per_launch < 16 || s->steps_per_launch > 8192 ||
+        (max_chain_steps(c) + s->steps_per_launch - 1) / s->steps_per_launch > kMaxChunks) {
+        c->err = "steps_per_launch out of range";
+        return CC_BADARG;
+    }
+    c->sched = *s;
+    c->sched.pipeline = s->pipeline != 0;
+    c->sched.use_graph = s->use_graph != 0;
+    drop_graph(c);
+    return CC_OK;
+}
+
+cc_status cc_autotune(cc_ctx *c, const double *x, int32_t flags, cc_schedule *out, double *times, int32_t ntimes,
+                      int32_t *ntried)
+{
+    if (!c || !x || (flags & ~(CC_TUNE_THREADS | CC_TUNE_STAGES)) != 0) return CC_BADARG;
+    if (max_chain_steps(c) < 64) {
+        c->err = "autotuning needs at least 64 steps per side";
+        return CC_BADARG;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->x, x, sizeof(double) * c->n, hipMemcpyHostToDevice, c->s0));
+    drop_graph(c);
+    std::vector<double> t;
+    const cc_schedule keep = c->sched;
+    cc_status st = CC_OK;
+    if (flags & CC_TUNE_THREADS)
+        dispatch_n(c->N, [&](auto Nc) { st = tune_threads_n<decltype(Nc)::value>(c, c->x, t); });
+    if (st == CC_OK && (flags & CC_TUNE_STAGES)) st = tune_stages(c, t);
+    if (st != CC_OK) {
+        c->sched = keep;
+        return st;
+    }
+    if (out) *out = c->sched;
+    if (times)
+        for (int i = 0; i < std::min<int>(ntimes, (int)t.size()); ++i) times[i] = t[i];
+    if (ntried) *ntried = (int32_t)t.size();
     return CC_OK;
 }
 
@@ -761,48 +1199,47 @@ cc_status cc_residual(cc_ctx *c, const double *x, double *F)
     const int n = c->n;
     HIPCHK(c, hipMemcpyAsync(c->x, x, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
     c->counters[0] = c->counters[1] = c->counters[2] = 0;
-    Plan P = make_plan(c, std::vector<int32_t>{-1}, false);
-    cc_status st = columns(c, P, main_buffers(c), c->x, 0.0, false, c->s0);
-    if (st != CC_OK) return st;
-    st = collect_status(c, P, main_buffers(c), c->s0, nullptr, true);
-    if (st != CC_OK) return st;
-    HIPCHK(c, hipMemcpyAsync(F, c->main.F, sizeof(double) * n, hipMemcpyDeviceToHost, c->s0));
+    time_reset(c);
+    const Counts k{1, 1, 2, 1};                 // the base column of the base plan
+    cc_status st = CC_OK;
+    dispatch_n(c->N, [&](auto Nc) {
+        st = launch_columns<decltype(Nc)::value>(c, c->base, k, c->x, 0.0, c->sched.threads_base, c->s0);
+    });
+    CCCHK(st);
+    CCCHK(readback_columns(c, c->base, k, true, c->s0));
+    HIPCHK(c, hipMemcpyAsync(F, c->base.F, sizeof(double) * n, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));
-    return CC_OK;
+    return eval_columns(c, c->base_plan, k, true, nullptr, true);
 }
 
 cc_status cc_jacobian(cc_ctx *c, const double *x, const double *F0, double *J, int32_t *col_status)
 {
     if (!c || !x || !F0 || !J) return CC_BADARG;
     const int n = c->n;
-    const bool mixed = c->p.precision == CC_MIXED;
-    const bool reuse = c->p.reuse_sides != 0;
     c->counters[0] = c->counters[1] = c->counters[2] = 0;
+    time_reset(c);
     HIPCHK(c, hipMemcpyAsync(c->x, x, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
     cc_status st;
-    if (mixed) {
-        std::vector<int32_t> tst, tcols;
-        st = mixed_jacobian(c, c->x, &tst, &tcols);
-        if (col_status) {
-            for (int i = 0; i < n; ++i) col_status[i] = 0;
-            for (size_t b = 0; b < tcols.size(); ++b) col_status[tcols[b]] = tst[b] ? CC_IRK_NOCONV : CC_OK;
-        }
+    if (c->p.precision == CC_MIXED) {
+        CCCHK(enqueue_mixed(c, c->x));
+        HIPCHK(c, hipMemcpyAsync(J, c->J, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->s0));
+        HIPCHK(c, hipStreamSynchronize(c->s0));
+        st = eval_mixed(c, col_status);
     } else {
-        // the base column is in the launch when its sides are reused; the
-        // difference quotient always uses the caller's F0 (NewtonSolver.cpp:684)
-        Plan P = make_plan(c, jacobian_cols(c, reuse), reuse);
-        st = columns(c, P, main_buffers(c), c->x, c->p.EpsNewton, false, c->s0);
-        if (st != CC_OK) return st;
-        const int ncols = (int)P.cols.size();
+        // the base column is part of the launch; the difference quotient uses
+        // the caller's F0 (NewtonSolver.cpp:684), stored after the last column
+        CCCHK(enqueue_fp64_columns(c, c->x));
+        const int ncols = (int)c->main_plan.cols.size();
         HIPCHK(c, hipMemcpyAsync(c->main.F + (size_t)ncols * n, F0, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
-        HIPCHK(c, up(P.cols, c->cols, c->s0));
-        hipLaunchKernelGGL(ccd::k_fd, dim3((ncols + 127) / 128, n), dim3(128), 0, c->s0, c->main.F, c->cols, ncols,
-                           ncols, n, c->p.EpsNewton, c->J);
+        hipLaunchKernelGGL(ccd::k_fd, dim3((ncols + 127) / 128, n), dim3(128), 0, c->s0, c->main.F, c->main.cols,
+                           ncols, ncols, n, c->p.EpsNewton, c->J);
         HIPCHK(c, hipGetLastError());
-        st = collect_status(c, P, main_buffers(c), c->s0, col_status, true);
+        CCCHK(readback_columns(c, c->main, counts_of(c->main_plan), false, c->s0));
+        HIPCHK(c, hipMemcpyAsync(J, c->J, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->s0));
+        HIPCHK(c, hipStreamSynchronize(c->s0));
+        st = eval_fp64(c, col_status);
     }
-    HIPCHK(c, hipMemcpyAsync(J, c->J, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->s0));
-    HIPCHK(c, hipStreamSynchronize(c->s0));
+    c->prof_pending = c->profile;
     return st;
 }
 
@@ -811,17 +1248,18 @@ cc_status cc_solve(cc_ctx *c, int32_t n, double *A, double *b, int32_t *info)
     if (!c || !A || !b || n < 1 || n > c->n) return CC_BADARG;
     HIPCHK(c, hipMemcpyAsync(c->J, A, sizeof(double) * n * n, hipMemcpyHostToDevice, c->s0));
     HIPCHK(c, hipMemcpyAsync(c->d, b, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
-    cc_status st = solve_device(c, n, c->J, c->d, c->dx, info, c->s0);
+    CCCHK(enqueue_solve(c, n, c->J, c->d, c->dx, c->s0));
     if (c->p.precision == CC_MIXED) {
         // FP32 factors, returned widened to double
         const unsigned g = (unsigned)(((size_t)n * n + 255) / 256);
         hipLaunchKernelGGL((ccd::k_cast_from<float>), dim3(g), dim3(256), 0, c->s0, c->A32, c->A, (size_t)n * n);
-        HIPCHK(c, hipMemcpyAsync(A, c->A, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->s0));
-    } else {
-        HIPCHK(c, hipMemcpyAsync(A, c->A, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->s0));
+        HIPCHK(c, hipGetLastError());
     }
-    if (st == CC_OK) HIPCHK(c, hipMemcpyAsync(b, c->dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipMemcpyAsync(A, c->A, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipMemcpyAsync(b, c->dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));
+    const cc_status st = eval_info(c, info);
+    if (st != CC_OK) HIPCHK(c, hipMemcpy(b, c->d, sizeof(double) * n, hipMemcpyDeviceToHost));   // b keeps its input
     return st;
 }
 
@@ -830,8 +1268,7 @@ cc_status cc_newton_step(cc_ctx *c, const double *x, double *F0, double *dx)
     if (!c || !x || !F0 || !dx) return CC_BADARG;
     const int n = c->n;
     HIPCHK(c, hipMemcpyAsync(c->x, x, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
-    cc_status st = newton_step_device(c, c->x, c->F0, c->dx, true);
-    if (st != CC_OK) return st;
+    CCCHK(newton_step_device(c, c->x, c->F0, c->dx));
     HIPCHK(c, hipMemcpyAsync(F0, c->F0, sizeof(double) * n, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipMemcpyAsync(dx, c->dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));
@@ -841,7 +1278,7 @@ cc_status cc_newton_step(cc_ctx *c, const double *x, double *F0, double *dx)
 cc_status cc_newton_step_device_async(cc_ctx *c, const double *d_x, double *d_F0, double *d_dx)
 {
     if (!c || !d_x || !d_F0 || !d_dx) return CC_BADARG;
-    return newton_step_device(c, d_x, d_F0, d_dx, true);
+    return newton_step_device(c, d_x, d_F0, d_dx);
 }
 
 cc_status cc_synchronize(cc_ctx *c)
@@ -861,22 +1298,12 @@ cc_status cc_fft_batched(cc_ctx *c, const double *in, double *out, int32_t batch
     HIPCHK(c, hipMalloc(&din, bytes));
     HIPCHK(c, hipMalloc(&dout, bytes));
     HIPCHK(c, hipMemcpyAsync(din, in, bytes, hipMemcpyHostToDevice, c->s0));
+    hipError_t e = hipSuccess;
     dispatch_n(N, [&](auto Nc) {
-        constexpr int NN = decltype(Nc)::value;
-        constexpr int T = tpb(NN);
-        if (fp32) {
-            if (sign > 0)
-                hipLaunchKernelGGL((ccd::k_fft<NN, T, float, 1>), dim3(batch), dim3(T), 0, c->s0, din, dout, c->tw32);
-            else
-                hipLaunchKernelGGL((ccd::k_fft<NN, T, float, -1>), dim3(batch), dim3(T), 0, c->s0, din, dout, c->tw32);
-        } else {
-            if (sign > 0)
-                hipLaunchKernelGGL((ccd::k_fft<NN, T, double, 1>), dim3(batch), dim3(T), 0, c->s0, din, dout, c->tw64);
-            else
-                hipLaunchKernelGGL((ccd::k_fft<NN, T, double, -1>), dim3(batch), dim3(T), 0, c->s0, din, dout, c->tw64);
-        }
+        e = Ops<decltype(Nc)::value>::fft(c->s0, fp32 != 0, sign, batch, din, dout,
+                                          fp32 ? (const void *)c->tw32 : (const void *)c->tw64);
     });
-    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, e);
     HIPCHK(c, hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));
     HIPCHK(c, hipFree(din));
@@ -909,21 +1336,21 @@ cc_status cc_rhs_eval(cc_ctx *c, const double *Y, const double *xs, double *dY, 
     HIPCHK(c, hipMemcpyAsync(dx_, xp.data(), sizeof(double) * 2 * batch, hipMemcpyHostToDevice, c->s0));
     HIPCHK(c, hipMemcpyAsync(dd, dp.data(), sizeof(double) * 2 * batch, hipMemcpyHostToDevice, c->s0));
     const unsigned g = (unsigned)((2 * nY + 255) / 256);
+    hipError_t e = hipSuccess;
     dispatch_n(N, [&](auto Nc) {
-        constexpr int NN = decltype(Nc)::value;
-        constexpr int T = tpb(NN);
+        using O = Ops<decltype(Nc)::value>;
         if (fp32) {
             hipLaunchKernelGGL((ccd::k_cast_to<float>), dim3(g), dim3(256), 0, c->s0, dtmp, (float *)dYin, 2 * nY);
-            hipLaunchKernelGGL((ccd::k_rhs_pairs<NN, T, float>), dim3(batch), dim3(T), 0, c->s0,
-                               (const cx<float> *)dYin, dx_, dd, (cx<float> *)dYout, c->tw32, c->invm32, c->p.Dim);
+            e = O::rhs_pairs(c->s0, true, batch, dYin, dx_, dd, dYout, c->tw32, c->invm32, c->p.Dim);
             hipLaunchKernelGGL((ccd::k_cast_from<float>), dim3(g), dim3(256), 0, c->s0, (const float *)dYout, dtmp,
                                2 * nY);
         } else {
-            hipLaunchKernelGGL((ccd::k_rhs_pairs<NN, T, double>), dim3(batch), dim3(T), 0, c->s0,
-                               (const cx<double> *)dtmp, dx_, dd, (cx<double> *)dYout, c->tw64, c->invm64, c->p.Dim);
-            (void)hipMemcpyAsync(dtmp, dYout, nY * sizeof(cx<double>), hipMemcpyDeviceToDevice, c->s0);
+            e = O::rhs_pairs(c->s0, false, batch, dtmp, dx_, dd, dYout, c->tw64, c->invm64, c->p.Dim);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(dtmp, dYout, nY * sizeof(cx<double>), hipMemcpyDeviceToDevice, c->s0);
         }
     });
+    HIPCHK(c, e);
     HIPCHK(c, hipGetLastError());
     std::vector<double> outp(2 * nY);
     HIPCHK(c, hipMemcpyAsync(outp.data(), dtmp, nY * sizeof(cx<double>), hipMemcpyDeviceToHost, c->s0));
@@ -970,12 +1397,12 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
     HIPCHK(c, hipMemcpyAsync(dst, start, sizeof(int32_t) * batch, hipMemcpyHostToDevice, c->s0));
     HIPCHK(c, hipMemcpyAsync(dns, ns.data(), sizeof(int32_t) * batch, hipMemcpyHostToDevice, c->s0));
     HIPCHK(c, hipMemcpyAsync(ddr, dr.data(), sizeof(int32_t) * batch, hipMemcpyHostToDevice, c->s0));
-    HIPCHK(c, hipMemsetAsync(dstat, 0, sizeof(int32_t) * batch, c->s0));
-    HIPCHK(c, hipMemsetAsync(dits, 0, sizeof(int64_t) * batch, c->s0));
+    CCCHK(fill<int32_t>(c, dstat, 0, batch, c->s0));
+    CCCHK(fill<int64_t>(c, dits, 0, batch, c->s0));
     const unsigned g = (unsigned)((2 * nY + 255) / 256);
+    hipError_t e = hipSuccess;
     dispatch_n(N, [&](auto Nc) {
-        constexpr int NN = decltype(Nc)::value;
-        constexpr int T = tpb(NN);
+        using O = Ops<decltype(Nc)::value>;
         auto run = [&](auto tag) {
             using R = decltype(tag);
             ccd::ChainParams<R> a{};
@@ -1001,10 +1428,12 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
             a.b1 = c->b[1];
             a.c0 = c->c[0];
             a.c1 = c->c[1];
-            for (int s0 = 0; s0 < maxsteps; s0 += kStepsPerLaunch) {
+            const int S = c->sched.steps_per_launch;
+            const int T = sizeof(R) == 8 ? c->sched.threads_base : O::kThreads;
+            for (int s0 = 0; s0 < maxsteps && e == hipSuccess; s0 += S) {
                 a.step_begin = s0;
-                a.step_end = std::min(maxsteps, s0 + kStepsPerLaunch);
-                hipLaunchKernelGGL((ccd::k_chain<NN, T, R>), dim3(batch), dim3(T), 0, c->s0, a);
+                a.step_end = std::min(maxsteps, s0 + S);
+                e = O::chain(c->s0, T, batch, a);
             }
         };
         if (fp32) {
@@ -1016,6 +1445,7 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
             run(double{});
         }
     });
+    HIPCHK(c, e);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(Y, dtmp, sizeof(cx<double>) * nY, hipMemcpyDeviceToHost, c->s0));
     std::vector<int32_t> stv(batch);
@@ -1033,24 +1463,48 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
     return CC_OK;
 }
 
+static cc_status pack_unpack(cc_ctx *c, const double *in, double *out, int32_t batch, bool pack)
+{
+    if (!c || !in || !out || batch < 1) return CC_BADARG;
+    const size_t nin = (size_t)batch * (pack ? 3 * c->N : c->n), nout = (size_t)batch * (pack ? c->n : 3 * c->N);
+    double *din = nullptr, *dout = nullptr;
+    HIPCHK(c, hipMalloc(&din, sizeof(double) * nin));
+    HIPCHK(c, hipMalloc(&dout, sizeof(double) * nout));
+    HIPCHK(c, hipMemcpyAsync(din, in, sizeof(double) * nin, hipMemcpyHostToDevice, c->s0));
+    hipError_t e = hipSuccess;
+    dispatch_n(c->N, [&](auto Nc) {
+        using O = Ops<decltype(Nc)::value>;
+        e = pack ? O::pack(c->s0, batch, din, c->tw64, dout) : O::unpack(c->s0, batch, din, c->tw64, dout);
+    });
+    HIPCHK(c, e);
+    HIPCHK(c, hipMemcpyAsync(out, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    HIPCHK(c, hipFree(din));
+    HIPCHK(c, hipFree(dout));
+    return CC_OK;
+}
+
+cc_status cc_pack(cc_ctx *c, const double *fields, double *x, int32_t batch)
+{
+    return pack_unpack(c, fields, x, batch, true);
+}
+
+cc_status cc_unpack(cc_ctx *c, const double *x, double *fields, int32_t batch)
+{
+    return pack_unpack(c, x, fields, batch, false);
+}
+
 cc_status cc_expansions(cc_ctx *c, const double *x, double *YL, double *YR)
 {
     if (!c || !x || !YL || !YR) return CC_BADARG;
     const int n = c->n, M = c->M;
     HIPCHK(c, hipMemcpyAsync(c->x, x, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
-    const int32_t cols[2] = {-1, -1}, ch[2] = {0, 1};
-    HIPCHK(c, hipMemcpyAsync(c->main.lcols, cols, sizeof(cols), hipMemcpyHostToDevice, c->s0));
-    HIPCHK(c, hipMemcpyAsync(c->main.lch, ch, sizeof(ch), hipMemcpyHostToDevice, c->s0));
-    cx<double> *Y = static_cast<cx<double> *>(c->main.Y);
-    dispatch_n(c->N, [&](auto Nc) {
-        constexpr int NN = decltype(Nc)::value;
-        constexpr int T = tpb(NN);
-        hipLaunchKernelGGL((ccd::k_boundary<NN, T, double, true>), dim3(1), dim3(T), 0, c->s0, c->x, c->main.lcols,
-                           c->main.lch, 0.0, c->p.XLeft, c->p.Dim, c->tw64, Y, c->main.delta);
-        hipLaunchKernelGGL((ccd::k_boundary<NN, T, double, false>), dim3(1), dim3(T), 0, c->s0, c->x, c->main.lcols,
-                           c->main.lch + 1, 0.0, c->p.XRight, c->p.Dim, c->tw64, Y, c->main.delta);
-    });
-    HIPCHK(c, hipGetLastError());
+    // the base column of the base plan: left chain 0, right chain 1
+    const Counts k{1, 1, 2, 1};
+    cc_status st = CC_OK;
+    dispatch_n(c->N, [&](auto Nc) { st = launch_prologue<decltype(Nc)::value>(c, c->base, k, c->x, 0.0, c->s0); });
+    CCCHK(st);
+    const cx<double> *Y = static_cast<const cx<double> *>(c->base.Y);
     HIPCHK(c, hipMemcpyAsync(YL, Y, sizeof(cx<double>) * M, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipMemcpyAsync(YR, Y + M, sizeof(cx<double>) * M, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));

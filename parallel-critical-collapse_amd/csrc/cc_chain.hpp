@@ -284,8 +284,13 @@ __device__ __forceinline__ RhsConst<R> make_const(double Dim, double delta)
     return K;
 }
 
+// Waves per SIMD the register budget is cut for: FP64 chains at four values
+// per thread fit 2 (256 VGPRs), at eight values 1; FP32 chains fit 3.
 template <int N, int T, typename R>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 2 : 3))) void k_chain(ChainParams<R> a)
+constexpr int chain_waves() { return sizeof(R) == 8 ? (N / T >= 8 ? 1 : 2) : 3; }
+
+template <int N, int T, typename R>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N, T, R>()))) void k_chain(ChainParams<R> a)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
     __shared__ cx<R> xb[2 * N];

@@ -1,8 +1,8 @@
-// cc_kernels.hpp -- per-column kernels around the chain kernel and the unit
-// transform kernel.  All of them use one workgroup per column (or signal).
+// cc_kernels.hpp -- length-N per-column kernels around the chain kernel and
+// the unit transform kernel.  All of them use one workgroup per column (or
+// signal).
 #pragma once
 #include "cc_chain.hpp"
-#include "cc_lu.hpp"
 #include "cc_spectral.hpp"
 
 namespace ccd {
@@ -75,15 +75,73 @@ __global__ __launch_bounds__(T) void k_mismatch(const cx<R> *Y, const int32_t *l
     state_to_packed<N, T>(S, mis, F + (size_t)b * n);
 }
 
-// J[j*n + c] = (F[c][j] - F[base][j]) / eps  (NewtonSolver.cpp:682-685)
-__global__ void k_fd(const double *F, const int32_t *cols, int ncols, int base, int n, double eps, double *J)
+// ICG::packSpectralFields (ICG.cpp:678-711) for a batch: fields[b] holds the
+// three real tau-series Up, psic, fc (N each); Z[b] receives the n-entry
+// Newton vector.  Forward transforms (1/N), halveModes, then odd modes
+// 1, 3, .. < N/4 of Up and psic and even modes 0, 2, .. < N/4 of fc as (re, im)
+// pairs; slot N/2+1 holds Re of fc's folded mode N/4 (its mode N/4 plus mode
+// 3N/4).  Slot N/2+2 is Re of fc's mode 2 (the caller overwrites it with
+// Delta, NewtonSolver.cpp:273-274).
+template <int N, int T>
+__global__ __launch_bounds__(T) void k_pack(const double *fields, const cx<double> *tw, double *Z)
 {
-    const int j = blockIdx.y;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= ncols || j >= n) return;
-    const int c = cols[b];
-    if (c < 0) return;
-    J[(size_t)j * n + c] = (F[(size_t)b * n + j] - F[(size_t)base * n + j]) / eps;
+    constexpr int P = N / T, n = 3 * N / 4;
+    __shared__ cx<double> xb[N];
+    __shared__ double bc[4];
+    const int t = threadIdx.x;
+    const double *f = fields + (size_t)blockIdx.x * 3 * N;
+    double *z = Z + (size_t)blockIdx.x * n;
+    Spec<N, T> S{t, xb, bc, tw, 0.0};
+    cx<double> U[P], V[P], E[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        U[p] = {f[t + T * p], 0.0};
+        V[p] = {f[N + t + T * p], 0.0};
+        E[p] = {f[2 * N + t + T * p], 0.0};
+    }
+    S.fwd(U);
+    S.fwd(V);
+    S.fwd(E);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int k = S.kidx(p);
+        if (k >= N / 4 + 1) continue;
+        if (k & 1) {
+            if (k < N / 4) {
+                z[k - 1] = U[p].x;
+                z[k] = U[p].y;
+                z[N / 4 + k - 1] = V[p].x;
+                z[N / 4 + k] = V[p].y;
+            }
+        } else if (k < N / 4) {
+            z[N / 2 + k] = E[p].x;
+            if (k != 0) z[N / 2 + k + 1] = E[p].y;
+        } else {                                  // k == N/4; 3N/4 sits P/2 slots later
+            z[N / 2 + 1] = E[p].x + E[p + P / 2].x;
+        }
+    }
+}
+
+// ICG::unpackSpectralFields (ICG.cpp:719-758) for a batch, reading the Delta
+// slot as zero as every call site does (NewtonSolver.cpp:408-409, 318-319):
+// Z[b] (n doubles) -> fields[b] = Up, psic, fc (N each).
+template <int N, int T>
+__global__ __launch_bounds__(T) void k_unpack(const double *Z, const cx<double> *tw, double *fields)
+{
+    constexpr int P = N / T, n = 3 * N / 4;
+    __shared__ cx<double> xb[N];
+    __shared__ double bc[4];
+    const int t = threadIdx.x;
+    Spec<N, T> S{t, xb, bc, tw, 0.0};
+    double Up[P], psic[P], fc[P];
+    unpack_fields<N, T>(S, Z + (size_t)blockIdx.x * n, -1, 0.0, Up, psic, fc);
+    double *f = fields + (size_t)blockIdx.x * 3 * N;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        f[t + T * p] = Up[p];
+        f[N + t + T * p] = psic[p];
+        f[2 * N + t + T * p] = fc[p];
+    }
 }
 
 // Batched length-N transform with the reference's scaling (unit entry point).
@@ -107,32 +165,6 @@ __global__ __launch_bounds__(T) void k_fft(const cx<double> *in, cx<double> *out
         if constexpr (SIGN > 0) r = cscale(r, (R)1 / (R)N);
         out[base + t + T * p] = {(double)r.x, (double)r.y};
     }
-}
-
-template <typename R>
-__global__ void k_cast_to(const double *in, R *out, size_t count)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) out[i] = (R)in[i];
-}
-
-template <typename R>
-__global__ void k_cast_from(const R *in, double *out, size_t count)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) out[i] = (double)in[i];
-}
-
-__global__ void k_negate_copy(const double *in, double *out, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = -in[i];
-}
-
-__global__ void k_axpy(double *x, const double *d, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] += d[i];
 }
 
 }  // namespace ccd

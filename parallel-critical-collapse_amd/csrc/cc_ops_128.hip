@@ -1,0 +1,4 @@
+// Kernels of transform length 128 (see cc_ops.hpp).
+#include "cc_ops_impl.hpp"
+
+template struct ccd::Ops<128>;

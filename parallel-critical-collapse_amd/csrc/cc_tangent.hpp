@@ -24,6 +24,7 @@
 // 8 length-N transforms per fixed-point iteration.
 #pragma once
 #include "cc_chain.hpp"
+#include "cc_spectral.hpp"
 
 namespace ccd {
 
@@ -32,6 +33,7 @@ struct TangentParams {
     const float *rec;         // base record
     const int64_t *rec_base;  // per chain: record slot of its step 0 (the base chain of its side)
     const int32_t *start, *nsteps, *dir;
+    const int32_t *base_status;  // status of the base chains (left, right), may be null
     const double *delta;      // delta[0]: Delta of the base; every tangent column shares it
                               // (the Delta column itself is a finite difference in FP64)
     const double *grid;
@@ -197,8 +199,11 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<float> (&dk)[2][N / T 
         }
 }
 
+// Register budget: 3 waves per SIMD (168 VGPRs) at four values per thread,
+// 2 (256 VGPRs) at eight, so that a launch of n two-wave workgroups still
+// fits the chip in one round.
 template <int N, int T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_tangent(TangentParams a)
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(N / T >= 8 ? 2 : 3))) void k_tangent(TangentParams a)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
     __shared__ cx<float> xb[2 * N];
@@ -208,6 +213,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_t
     const int t = threadIdx.x;
     const int ch = blockIdx.x;
     if (a.status[ch] != 0) return;
+    const int dir = a.dir[ch];
+    // a failed base chain leaves no record to linearise around
+    if (a.base_status && a.base_status[dir > 0 ? 0 : 1] != 0) return;
     const int nst = a.nsteps[ch];
     const int s_begin = a.step_begin;
     const int s_end = min(a.step_end, nst);
@@ -216,7 +224,6 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_t
 #pragma unroll
     for (int h = 0; h < H; ++h) ys[0][t + T * h] = Yg[t + T * h];
     const RhsConst<float> K = make_const<N, float>(a.Dim, a.delta[0]);
-    const int dir = a.dir[ch];
     int idx = a.start[ch] + dir * s_begin;
     int64_t its_sum = 0;
     int32_t st = 0;

@@ -1,0 +1,309 @@
+// newton.cpp -- SimulationConfig and NewtonSolver over the C-ABI (see newton.hpp).
+#include "newton.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+
+namespace cchost {
+
+namespace {
+
+const Json &need(const Json &j, const char *key)
+{
+    const Json &v = j[key];
+    if (v.is_null()) throw std::invalid_argument(std::string("simulation input lacks key '") + key + "'");
+    return v;
+}
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+SimulationConfig SimulationConfig::from_json(const Json &j)
+{
+    SimulationConfig c;
+    c.Ntau = (int)need(j, "Ntau").number();
+    c.Dim = need(j, "Dim").number();
+    c.XLeft = need(j, "XLeft").number();
+    c.XMid = need(j, "XMid").number();
+    c.XRight = need(j, "XRight").number();
+    c.EpsNewton = need(j, "EpsNewton").number();
+    c.PrecisionNewton = need(j, "PrecisionNewton").number();
+    c.SlowError = need(j, "SlowError").number();
+    c.MaxIterNewton = (int)need(j, "MaxIterNewton").number();
+    c.Verbose = need(j, "Verbose").boolean();
+    c.Debug = need(j, "Debug").boolean();
+    c.Converged = need(j, "Converged").boolean();
+    c.NLeft = (int)need(j, "NLeft").number();
+    c.NRight = (int)need(j, "NRight").number();
+    c.PrecisionIRK = need(j, "PrecisionIRK").number();
+    c.MaxIterIRK = (int)need(j, "MaxIterIRK").number();
+    c.SchemeIRK = (int)need(j, "SchemeIRK").number();
+    if (c.SchemeIRK < 1 || c.SchemeIRK > 3) throw std::invalid_argument("Wrong IRK Scheme stage supplied!");
+    if (!j["DebugNx"].is_null()) c.DebugNx = (int)j["DebugNx"].number();
+    if (!j["DebugNtau"].is_null()) c.DebugNtau = (int)j["DebugNtau"].number();
+    const Json &ic = j["Initial_Conditions"];
+    if (!ic["Delta"].is_null()) {
+        c.has_delta = true;
+        c.Delta = ic["Delta"].number();
+    }
+    if (!ic["fc"].is_null()) c.fc = ic["fc"].numbers();
+    if (!ic["psic"].is_null()) c.psic = ic["psic"].numbers();
+    if (!ic["Up"].is_null()) c.Up = ic["Up"].numbers();
+    return c;
+}
+
+SimulationConfig SimulationConfig::load(const std::string &path)
+{
+    std::ifstream f(path);
+    if (!f) throw std::invalid_argument("Invalid simulation input path!");
+    std::stringstream ss;
+    ss << f.rdbuf();
+    return from_json(Json::parse(ss.str()));
+}
+
+Json SimulationConfig::to_json() const
+{
+    Json j = Json::object();
+    j["Ntau"] = Ntau;
+    j["Dim"] = Dim;
+    j["XLeft"] = XLeft;
+    j["XMid"] = XMid;
+    j["XRight"] = XRight;
+    j["EpsNewton"] = EpsNewton;
+    j["PrecisionNewton"] = PrecisionNewton;
+    j["MaxIterNewton"] = MaxIterNewton;
+    j["Verbose"] = Verbose;
+    j["SlowError"] = SlowError;
+    j["NLeft"] = NLeft;
+    j["NRight"] = NRight;
+    j["SchemeIRK"] = SchemeIRK;
+    j["PrecisionIRK"] = PrecisionIRK;
+    j["MaxIterIRK"] = MaxIterIRK;
+    j["Converged"] = Converged;
+    j["Debug"] = Debug;
+    Json ic = Json::object();
+    ic["Delta"] = has_delta ? Json(Delta) : Json();
+    ic["fc"] = fc.empty() ? Json() : Json(fc);
+    ic["psic"] = psic.empty() ? Json() : Json(psic);
+    ic["Up"] = Up.empty() ? Json() : Json(Up);
+    j["Initial_Conditions"] = ic;
+    return j;
+}
+
+NewtonSolver::NewtonSolver(const SimulationConfig &config, const SolverOptions &opt, bool benchmark)
+    : cfg_(config), opt_(opt), benchmark_(benchmark)
+{
+    N_ = cfg_.Ntau;
+    n_ = 3 * N_ / 4;
+    if (cfg_.SchemeIRK != 2)
+        throw std::invalid_argument("the device path integrates with SchemeIRK 2 (Gauss-Legendre, 2 stages)");
+    if (!cfg_.has_delta || (int)cfg_.fc.size() != N_ || (int)cfg_.psic.size() != N_ || (int)cfg_.Up.size() != N_)
+        throw std::invalid_argument("Initial_Conditions must hold Delta and fc, psic, Up of length Ntau");
+    cc_problem p{};
+    p.Ntau = N_;
+    p.SchemeIRK = cfg_.SchemeIRK;
+    p.MaxIterIRK = cfg_.MaxIterIRK;
+    p.NLeft = cfg_.NLeft;
+    p.NRight = cfg_.NRight;
+    p.Dim = cfg_.Dim;
+    p.XLeft = cfg_.XLeft;
+    p.XMid = cfg_.XMid;
+    p.XRight = cfg_.XRight;
+    p.PrecisionIRK = cfg_.PrecisionIRK;
+    p.EpsNewton = cfg_.EpsNewton;
+    p.precision = opt_.policy == Policy::Mixed ? CC_MIXED : CC_FP64;
+    p.reuse_sides = opt_.reuse_sides ? 1 : 0;
+    const cc_status st = cc_ctx_create(&p, &ctx_);
+    if (st != CC_OK) throw SolverError(st, "cc_ctx_create failed (unsupported Ntau or device error)");
+    Up_ = cfg_.Up;
+    psic_ = cfg_.psic;
+    fc_ = cfg_.fc;
+    Delta_ = cfg_.Delta;
+}
+
+NewtonSolver::~NewtonSolver()
+{
+    if (ctx_) cc_ctx_destroy(ctx_);
+}
+
+cc_schedule NewtonSolver::schedule() const
+{
+    cc_schedule s{};
+    cc_get_schedule(ctx_, &s);
+    return s;
+}
+
+void NewtonSolver::check(cc_status st, const char *what) const
+{
+    if (st == CC_OK) return;
+    std::string msg = std::string(what) + ": " + cc_last_error(ctx_);
+    if (st == CC_SINGULAR) msg = "LAPACKE_dgesv failed: " + std::string(cc_last_error(ctx_));
+    throw SolverError(st, msg);
+}
+
+std::vector<double> NewtonSolver::pack() const
+{
+    std::vector<double> fields((size_t)3 * N_), x(n_);
+    std::copy(Up_.begin(), Up_.end(), fields.begin());
+    std::copy(psic_.begin(), psic_.end(), fields.begin() + N_);
+    std::copy(fc_.begin(), fc_.end(), fields.begin() + 2 * N_);
+    check(cc_pack(ctx_, fields.data(), x.data(), 1), "cc_pack");
+    return x;
+}
+
+void NewtonSolver::unpack(const std::vector<double> &x)
+{
+    std::vector<double> fields((size_t)3 * N_);
+    check(cc_unpack(ctx_, x.data(), fields.data(), 1), "cc_unpack");
+    Up_.assign(fields.begin(), fields.begin() + N_);
+    psic_.assign(fields.begin() + N_, fields.begin() + 2 * N_);
+    fc_.assign(fields.begin() + 2 * N_, fields.end());
+}
+
+// NewtonSolver.cpp:763-768
+double NewtonSolver::l2norm(const std::vector<double> &v)
+{
+    double s = 0.0;
+    for (double e : v) s += e * e;
+    return std::sqrt(s / (double)v.size());
+}
+
+// NewtonSolver.cpp:775-816
+void NewtonSolver::write_final_output(size_t newton_its, double mismatch)
+{
+    Json &r = result_;
+    r = Json::object();
+    r["Ntau"] = cfg_.Ntau;
+    r["Dim"] = cfg_.Dim;
+    r["XLeft"] = cfg_.XLeft;
+    r["XMid"] = cfg_.XMid;
+    r["XRight"] = cfg_.XRight;
+    r["EpsNewton"] = cfg_.EpsNewton;
+    r["PrecisionNewton"] = cfg_.PrecisionNewton;
+    r["SlowError"] = cfg_.SlowError;
+    r["MaxIterNewton"] = cfg_.MaxIterNewton;
+    r["Verbose"] = cfg_.Verbose;
+    r["Debug"] = cfg_.Debug;
+    if (cfg_.Debug) {
+        r["DebugNx"] = cfg_.DebugNx;
+        r["DebugNtau"] = cfg_.DebugNtau;
+    }
+    r["Converged"] = true;
+    r["NLeft"] = cfg_.NLeft;
+    r["NRight"] = cfg_.NRight;
+    r["SchemeIRK"] = cfg_.SchemeIRK;
+    r["PrecisionIRK"] = cfg_.PrecisionIRK;
+    r["MaxIterIRK"] = cfg_.MaxIterIRK;
+    r["IterNewton"] = (long)newton_its;
+    r["mismatchNorm"] = mismatch;
+    Json ic = Json::object();
+    ic["Delta"] = Delta_;
+    ic["fc"] = Json(fc_);
+    ic["psic"] = Json(psic_);
+    ic["Up"] = Json(Up_);
+    r["Initial_Conditions"] = ic;
+    if (opt_.verbose)
+        std::cout << "Final result stored in simulation dictionary for dimension D = " << cfg_.Dim << "." << std::endl
+                  << std::endl;
+}
+
+// NewtonSolver.cpp:260-393 (serial/OpenMP control flow).  One Newton step of
+// the device path returns F(x) together with the solved update, so the
+// Jacobian of the converging iteration is computed and discarded.
+Json NewtonSolver::run(Json *bench)
+{
+    const double t_all = now_s();
+    const int dslot = N_ / 2 + 2;
+    auto say = [&](const std::string &s) {
+        if (opt_.verbose) std::cout << s << std::endl;
+    };
+    history_.clear();
+    if (!cfg_.Converged) {
+        double fac = 1.0, err = 1.0, errOld = 1.0;
+        int increase = 0;
+        x_ = pack();
+        x_[dslot] = Delta_;
+        std::vector<double> xold = x_, F0(n_), dx(n_);
+        if (opt_.autotune) {
+            const double t = now_s();
+            cc_schedule s{};
+            check(cc_autotune(ctx_, x_.data(), CC_TUNE_THREADS | CC_TUNE_STAGES, &s, nullptr, 0, nullptr),
+                  "cc_autotune");
+            if (bench) (*bench)["AutotuneTime"] = now_s() - t;
+        }
+        if (opt_.use_graph) {
+            cc_schedule s = schedule();
+            s.use_graph = 1;
+            check(cc_set_schedule(ctx_, &s), "cc_set_schedule");
+        }
+        bool converged = false;
+        for (size_t its = 0; its < (size_t)cfg_.MaxIterNewton; ++its) {
+            const double t_step = now_s();
+            say("Newton iteration: " + std::to_string(its + 1));
+            errOld = err;
+            check(cc_newton_step(ctx_, x_.data(), F0.data(), dx.data()), "Newton step");
+            err = l2norm(F0);
+            history_.push_back(err);
+            if (opt_.verbose) std::cout << "Mismatch norm: " << err << std::endl;
+            if (err < cfg_.PrecisionNewton) {
+                converged = true;
+                say("The solution has converged!\n");
+                Delta_ = x_[dslot];
+                std::vector<double> xv = x_;
+                unpack(xv);                      // reads the Delta slot as zero
+                its_ = its;
+                err_ = err;
+                write_final_output(its, err);
+                break;
+            }
+            if (std::log10(err) > std::log10(errOld)) ++increase;
+            if (increase >= 5) {
+                converged = true;
+                Delta_ = xold[dslot];
+                xold[dslot] = 0.0;
+                unpack(xold);
+                its_ = its;
+                err_ = errOld;
+                write_final_output(its, errOld);
+                std::cerr << "Mismatch increased – terminating Newton.\n";
+                break;
+            }
+            fac = std::min(1.0, cfg_.SlowError / err);
+            if (std::log10(err) < std::log10(errOld)) {
+                xold = x_;
+                increase = 0;
+            }
+            for (int i = 0; i < n_; ++i) x_[i] += fac * dx[i];
+            if (bench) (*bench)["NewtonStep_" + std::to_string(its + 1)] = now_s() - t_step;
+        }
+        if (!converged) {
+            std::ostringstream m;
+            m << "Newton method did not converge in " << cfg_.MaxIterNewton << " iterations. For dimension: "
+              << cfg_.Dim;
+            throw SolverError(CC_IRK_NOCONV, m.str());
+        }
+    } else {
+        x_ = pack();
+        x_[dslot] = Delta_;
+        std::vector<double> F(n_);
+        check(cc_residual(ctx_, x_.data(), F.data()), "shoot");
+        const double err = l2norm(F);
+        history_.push_back(err);
+        say("The solution is already marked as converged!");
+        its_ = 0;
+        err_ = err;
+        write_final_output(0, err);
+    }
+    if (bench) (*bench)["OverallTime"] = now_s() - t_all;
+    return result_;
+}
+
+}  // namespace cchost

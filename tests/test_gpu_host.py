@@ -1,0 +1,93 @@
+"""The C++ NewtonSolver driver (cc_newton) on the GPU: same control flow as
+NewtonSolver::run (NewtonSolver.cpp:260-393), checked against the same loop
+written here over the Python binding of the same C-ABI (bit-identical: both
+call the same kernels and do the same host arithmetic), and its result
+dictionary (NewtonSolver.cpp:775-816)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from _helpers import ROOT, W, cc
+from test_host import config_dict
+
+pytestmark = pytest.mark.gpu
+CLI = os.path.join(ROOT, "parallel-critical-collapse_amd", "cc_newton")
+
+
+def rms(v):
+    """computeL2Norm (NewtonSolver.cpp:763-768) with the same left-to-right sum as the driver."""
+    s = 0.0
+    for e in v:
+        s += float(e) * float(e)
+    return float(np.sqrt(s / len(v)))
+
+
+def newton_loop(shape, x0, tol, max_its, slow):
+    ctx = cc.Context(cc.Problem.from_shape(shape, precision=cc.CC_MIXED))
+    x = x0.copy()
+    hist = []
+    for its in range(max_its):
+        F0, dx = ctx.newton_step(x)
+        err = rms(F0)
+        hist.append(err)
+        if err < tol:
+            return x, its, err, hist, ctx
+        x = x + min(1.0, slow / err) * dx
+    raise AssertionError(f"no convergence: {hist}")
+
+
+@pytest.fixture(scope="module")
+def anchored_config(tmp_path_factory):
+    """The shipped near-solution at Ntau=128 on the 1000-point grid."""
+    shape = W.Shape(Ntau=128, X=1000)
+    xa, Delta = W.anchor(128)
+    ctx = cc.Context(cc.Problem.from_shape(shape))
+    Up, psic, fc = ctx.unpack(xa)
+    d = config_dict(shape, Up, psic, fc, Delta, PrecisionNewton=1e-9, MaxIterNewton=8)
+    p = tmp_path_factory.mktemp("host") / "sim.json"
+    p.write_text(json.dumps(d))
+    x0 = ctx.pack(Up, psic, fc)
+    x0[128 // 2 + 2] = Delta
+    return shape, p, d, x0
+
+
+def test_cc_newton_matches_loop(anchored_config, tmp_path):
+    shape, p, d, x0 = anchored_config
+    out = tmp_path / "result.json"
+    print("cc_newton mixed Ntau=128 X=1000", flush=True)
+    r = subprocess.run([CLI, "-i", str(p), "-o", str(out), "--precision", "mixed"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    res = json.loads(out.read_text())
+    x, its, err, hist, ctx = newton_loop(shape, x0, d["PrecisionNewton"], d["MaxIterNewton"], d["SlowError"])
+    assert res["Converged"] is True
+    assert res["IterNewton"] == its
+    assert res["mismatchNorm"] == err
+    assert res["Initial_Conditions"]["Delta"] == x[128 // 2 + 2]
+    for k in ("Ntau", "Dim", "XMid", "NLeft", "NRight", "PrecisionNewton", "SchemeIRK"):
+        assert res[k] == d[k]
+    # the written fields pack back to the converged Newton vector
+    ic = res["Initial_Conditions"]
+    xb = ctx.pack(np.array(ic["Up"]), np.array(ic["psic"]), np.array(ic["fc"]))
+    xb[128 // 2 + 2] = ic["Delta"]
+    assert np.max(np.abs(xb - x)) < 1e-13
+    assert np.sqrt(np.mean(ctx.residual(xb) ** 2)) < d["PrecisionNewton"]
+
+
+def test_cc_newton_already_converged(anchored_config, tmp_path):
+    shape, p, d, x0 = anchored_config
+    d2 = dict(d, Converged=True)
+    p2 = tmp_path / "conv.json"
+    p2.write_text(json.dumps(d2))
+    out = tmp_path / "result.json"
+    r = subprocess.run([CLI, "-i", str(p2), "-o", str(out), "--quiet"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = json.loads(out.read_text())
+    ctx = cc.Context(cc.Problem.from_shape(shape))
+    F = ctx.residual(x0)
+    assert res["IterNewton"] == 0
+    assert res["mismatchNorm"] == rms(F)
