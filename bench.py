@@ -125,6 +125,68 @@ def cpu_baseline(shape, threads, budget_s):
     }
 
 
+def rms(v):
+    """computeL2Norm (NewtonSolver.cpp:763-768)."""
+    return float(np.sqrt(np.mean(np.asarray(v) ** 2)))
+
+
+def newton_study(cc, shape, policies, tol, max_its, tune):
+    """Newton from the seeded input to mismatch < tol under each precision
+    policy, with the reference's damping fac = min(1, SlowError/err) and its
+    divergence guard (NewtonSolver.cpp:298-358): iterations, time per
+    iteration, converged Delta, and Delta's relative difference between
+    policies."""
+    W = importlib.import_module("parallel-critical-collapse_amd.workload")
+    names = {"fp64": cc.CC_FP64, "mixed": cc.CC_MIXED, "fp64_tangent": cc.CC_FP64_TANGENT}
+    dslot = shape.Ntau // 2 + 2
+    out = {}
+    for name in policies:
+        ctx = cc.Context(cc.Problem.from_shape(shape, precision=names[name]))
+        x = W.initial_x(shape)
+        if tune:
+            ctx.autotune(x)
+        hist, secs = [], []
+        err, err_old, increase, reached = 1.0, 1.0, 0, None
+        for its in range(max_its):
+            t = time.perf_counter()
+            F0, dx = ctx.newton_step(x)
+            err_old, err = err, rms(F0)
+            hist.append(err)
+            log(f"[newton] {name} iteration {its + 1}: mismatch {err:.3e}")
+            if err < tol:
+                reached = its
+                secs.append(time.perf_counter() - t)
+                break
+            increase = increase + 1 if err > err_old else 0
+            if increase >= 5:
+                secs.append(time.perf_counter() - t)
+                break
+            x = x + min(1.0, shape.SlowError / err) * dx
+            secs.append(time.perf_counter() - t)
+        out[name] = {"iterations_to_tol": reached, "mismatch": hist, "final_mismatch": hist[-1],
+                     "Delta": float(x[dslot]), "s_per_iteration": float(np.mean(secs)),
+                     "total_s": float(np.sum(secs)), "schedule": ctx.schedule().as_dict()}
+        ctx.close()
+    ref = "fp64_tangent" if "fp64_tangent" in out else None
+    if ref:
+        for name, r in out.items():
+            r["Delta_rel_diff_vs_fp64_tangent"] = abs(r["Delta"] - out[ref]["Delta"]) / abs(out[ref]["Delta"])
+    return out
+
+
+def copy_calibration(nbytes):
+    """A plain device copy of nbytes (torch's copy kernel), for scaling the
+    TCC FETCH_SIZE / WRITE_SIZE counters of a --pmc run to bytes."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "read": n * 4, "written": n * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,7 +199,26 @@ def main():
     ap.add_argument("--schedule", default="", help='fixed schedule fields as JSON, e.g. \'{"threads_tangent": 128}\'; '
                     "skips the autotuner")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--newton", action="store_true",
+                    help="Newton to convergence under each of --policies instead of the step benchmark")
+    ap.add_argument("--policies", default="mixed,fp64_tangent,fp64")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--max-its", type=int, default=12)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.newton:
+        cc = importlib.import_module("parallel-critical-collapse_amd")
+        W = importlib.import_module("parallel-critical-collapse_amd.workload")
+        shape = W.config(args.config)
+        res = newton_study(cc, shape, args.policies.split(","), args.tol, args.max_its, not args.no_tune)
+        print(json.dumps({"workload": f"newton Ntau={shape.Ntau} x_points={shape.X} n={shape.n} Dim={shape.Dim} "
+                                      f"XMid={shape.XMid} scale={shape.scale}", "tol": args.tol,
+                          "policies": res}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the column shards of a multi-GPU run are described in DESIGN.md)", file=sys.stderr)

@@ -37,8 +37,11 @@ typedef enum cc_status {
 } cc_status;
 
 typedef enum cc_precision {
-    CC_FP64 = 0,            /* every shoot in FP64: the reference's arithmetic */
-    CC_MIXED = 1            /* perturbed shoots + LU in FP32, FP64 residual, FP64 refinement */
+    CC_FP64 = 0,            /* every column a forward difference of two FP64 shoots: the reference's arithmetic */
+    CC_MIXED = 1,           /* FP64 base shoot, F(x) and Delta column; every other column the FP32
+                               tangent-linear derivative of the shoot; FP32 LU + FP64 refinement */
+    CC_FP64_TANGENT = 2     /* as CC_MIXED with the tangent-linear columns and the LU in FP64:
+                               an accurate all-FP64 Jacobian (no counterpart in the reference) */
 } cc_precision;
 
 /* Problem description: the SimulationConfig keys the path reads

@@ -100,17 +100,17 @@ struct cc_ctx {
     ColumnBuffers base;                  // base + Delta column (residual, mixed policy)
     Plan main_plan, base_plan;
     // tangent-linear columns of the mixed policy
-    float *rec = nullptr;                // base-trajectory record, X steps
+    void *rec = nullptr;                 // base-trajectory record, X steps (float, or double in CC_FP64_TANGENT)
     int64_t *rec_base_b = nullptr;       // record slots of the base launch's chains
     int64_t *rec_base_t = nullptr;       // record slots of the tangent chains
-    cx<float> *dY = nullptr;             // [n-1][M] tangent states
+    void *dY = nullptr;                  // [n-1][M] tangent states (cx<float>, or cx<double>)
     int32_t *t_start = nullptr, *t_nsteps = nullptr, *t_dir = nullptr, *t_status = nullptr, *t_fail = nullptr;
     int32_t *t_cols = nullptr, *t_lcols = nullptr, *t_rcols = nullptr, *t_lch = nullptr, *t_rch = nullptr;
     int32_t *t_lidx = nullptr, *t_ridx = nullptr;
     int64_t *t_its = nullptr;
     std::vector<int32_t> t_cols_h, t_nsteps_h;
     int t_nl = 0, t_nr = 0;
-    double tangent_rtol = 1e-6, tangent_h = 1e-5;
+    double tangent_rtol = 1e-6, tangent_h = 1e-5;   // CC_FP64_TANGENT: rtol 1e-13
     int maxcols = 0, maxchains = 0;
     cc_schedule sched{};
     void *hblock = nullptr;
@@ -483,12 +483,12 @@ cc_status eval_info(cc_ctx *c, int32_t *info_out)
 // chunk waits on the event of its base chunk, so base chunk k+1 overlaps
 // tangent chunk k.
 // ---------------------------------------------------------------------------
-template <int N>
-ccd::TangentParams tangent_params(const cc_ctx *c)
+template <int N, typename R>
+ccd::TangentParams<R> tangent_params(const cc_ctx *c)
 {
-    ccd::TangentParams a{};
-    a.dY = c->dY;
-    a.rec = c->rec;
+    ccd::TangentParams<R> a{};
+    a.dY = static_cast<cx<R> *>(c->dY);
+    a.rec = static_cast<const R *>(c->rec);
     a.rec_base = c->rec_base_t;
     a.start = c->t_start;
     a.nsteps = c->t_nsteps;
@@ -499,8 +499,13 @@ ccd::TangentParams tangent_params(const cc_ctx *c)
     a.status = c->t_status;
     a.fail_idx = c->t_fail;
     a.its = c->t_its;
-    a.tw = c->tw32;
-    a.invm = c->invm32;
+    if constexpr (sizeof(R) == 8) {
+        a.tw = c->tw64;
+        a.invm = c->invm64;
+    } else {
+        a.tw = c->tw32;
+        a.invm = c->invm32;
+    }
     a.maxIts = c->p.MaxIterIRK;
     a.Dim = c->p.Dim;
     a.rtol = c->tangent_rtol;
@@ -522,18 +527,20 @@ cc_status tangent_prologue(cc_ctx *c, const double *d_x, hipStream_t s)
     CCCHK(fill<int32_t>(c, c->t_status, 0, nt, s));
     CCCHK(fill<int32_t>(c, c->t_fail, -1, nt, s));
     CCCHK(fill<int64_t>(c, c->t_its, 0, nt, s));
+    const bool t64 = c->p.precision == CC_FP64_TANGENT;
     if (c->t_nl)
-        HIPCHK(c, Ops<N>::boundary_tangent(s, true, c->t_nl, d_x, c->t_lcols, c->t_lch, c->tangent_h, c->p.XLeft,
-                                           c->p.Dim, c->tw64, c->dY));
+        HIPCHK(c, Ops<N>::boundary_tangent(s, true, t64, c->t_nl, d_x, c->t_lcols, c->t_lch, c->tangent_h,
+                                           c->p.XLeft, c->p.Dim, c->tw64, c->dY));
     if (c->t_nr)
-        HIPCHK(c, Ops<N>::boundary_tangent(s, false, c->t_nr, d_x, c->t_rcols, c->t_rch, c->tangent_h, c->p.XRight,
-                                           c->p.Dim, c->tw64, c->dY));
+        HIPCHK(c, Ops<N>::boundary_tangent(s, false, t64, c->t_nr, d_x, c->t_rcols, c->t_rch, c->tangent_h,
+                                           c->p.XRight, c->p.Dim, c->tw64, c->dY));
     return CC_OK;
 }
 
-template <int N>
-cc_status enqueue_mixed_n(cc_ctx *c, const double *d_x)
+template <int N, typename R>
+cc_status enqueue_tangent_n(cc_ctx *c, const double *d_x)
 {
+    constexpr bool t64 = sizeof(R) == 8;
     const int n = c->n;
     const int nt = (int)c->t_cols_h.size();
     const bool pipe = c->sched.pipeline != 0;
@@ -551,11 +558,11 @@ cc_status enqueue_mixed_n(cc_ctx *c, const double *d_x)
     ccd::ChainParams<double> ab = chain_params<N>(c, c->base);
     ab.rec = c->rec;
     ab.rec_base = c->rec_base_b;
-    ccd::TangentParams at = tangent_params<N>(c);
+    ccd::TangentParams<R> at = tangent_params<N, R>(c);
     for (int k = 0; k < nchunks; ++k) {
         ab.step_begin = at.step_begin = k * S;
         ab.step_end = at.step_end = std::min(maxsteps, (k + 1) * S);
-        HIPCHK(c, Ops<N>::chain(sb, c->sched.threads_base, kb.nch, ab));
+        HIPCHK(c, Ops<N>::chain(sb, c->sched.threads_base, kb.nch, ab, t64));
         if (pipe) {
             HIPCHK(c, hipEventRecord(c->chunk_ev[k], sb));
             HIPCHK(c, hipStreamWaitEvent(s0, c->chunk_ev[k], 0));
@@ -574,22 +581,28 @@ cc_status enqueue_mixed_n(cc_ctx *c, const double *d_x)
         HIPCHK(c, hipEventRecord(c->ev_join, sb));
         HIPCHK(c, hipStreamWaitEvent(s0, c->ev_join, 0));
     }
-    HIPCHK(c, Ops<N>::tangent_column(s0, nt, c->dY, c->t_cols, c->t_lidx, c->t_ridx, c->tw64, c->J));
+    HIPCHK(c, Ops<N>::tangent_column(s0, t64, nt, c->dY, c->t_cols, c->t_lidx, c->t_ridx, c->tw64, c->J));
     CCCHK(readback_columns(c, c->base, kb, true, s0));
     HIPCHK(c, hipMemcpyAsync(c->hs.tstatus, c->t_status, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, s0));
     HIPCHK(c, hipMemcpyAsync(c->hs.tits, c->t_its, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, s0));
     return CC_OK;
 }
 
-cc_status enqueue_mixed(cc_ctx *c, const double *d_x)
+cc_status enqueue_tangent(cc_ctx *c, const double *d_x)
 {
     cc_status st = CC_OK;
-    dispatch_n(c->N, [&](auto Nc) { st = enqueue_mixed_n<decltype(Nc)::value>(c, d_x); });
+    const bool t64 = c->p.precision == CC_FP64_TANGENT;
+    dispatch_n(c->N, [&](auto Nc) {
+        constexpr int NN = decltype(Nc)::value;
+        st = t64 ? enqueue_tangent_n<NN, double>(c, d_x) : enqueue_tangent_n<NN, float>(c, d_x);
+    });
     return st;
 }
 
-// Statuses and counters of a mixed Jacobian (after the stream synchronised).
-cc_status eval_mixed(cc_ctx *c, int32_t *col_status)
+bool tangent_policy(const cc_ctx *c) { return c->p.precision == CC_MIXED || c->p.precision == CC_FP64_TANGENT; }
+
+// Statuses and counters of a tangent-linear Jacobian (after the stream synchronised).
+cc_status eval_tangent(cc_ctx *c, int32_t *col_status)
 {
     const cc_status sb = eval_columns(c, c->base_plan, counts_of(c->base_plan), true, nullptr, true);
     const int nt = (int)c->t_cols_h.size();
@@ -650,8 +663,8 @@ cc_status eval_fp64(cc_ctx *c, int32_t *col_status)
 cc_status enqueue_newton(cc_ctx *c, const double *d_x, double *d_F0, double *d_dx)
 {
     const int n = c->n;
-    if (c->p.precision == CC_MIXED) {
-        CCCHK(enqueue_mixed(c, d_x));
+    if (tangent_policy(c)) {
+        CCCHK(enqueue_tangent(c, d_x));
         HIPCHK(c, hipMemcpyAsync(d_F0, c->base.F, sizeof(double) * n, hipMemcpyDeviceToDevice, c->s0));
     } else {
         CCCHK(enqueue_fp64_columns(c, d_x));
@@ -681,7 +694,7 @@ cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double 
     if (c->sched.use_graph) {
         if (!c->gexec || c->gkey[0] != d_x || c->gkey[1] != d_F0 || c->gkey[2] != d_dx) {
             drop_graph(c);
-            if (c->sched.pipeline && c->p.precision == CC_MIXED) {
+            if (c->sched.pipeline && tangent_policy(c)) {
                 const int S = c->sched.steps_per_launch;
                 CCCHK(ensure_chunk_events(c, (max_chain_steps(c) + S - 1) / S));
             }
@@ -709,7 +722,7 @@ cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double 
         c->prof_pending = c->profile;
     }
     HIPCHK(c, hipStreamSynchronize(c->s0));
-    const cc_status st = c->p.precision == CC_MIXED ? eval_mixed(c, nullptr) : eval_fp64(c, nullptr);
+    const cc_status st = tangent_policy(c) ? eval_tangent(c, nullptr) : eval_fp64(c, nullptr);
     CCCHK(st);
     return eval_info(c, nullptr);
 }
@@ -730,11 +743,12 @@ template <int N>
 cc_status tune_threads_n(cc_ctx *c, const double *d_x, std::vector<double> &times)
 {
     using O = Ops<N>;
+    const bool mixed = c->p.precision == CC_MIXED;
+    const bool t64 = c->p.precision == CC_FP64_TANGENT;     // FP64 record and tangent: default threads only
     std::vector<int> Ts = {O::kThreads};
-    if (O::kAltThreads) Ts.push_back(O::kAltThreads);
+    if (O::kAltThreads && !t64) Ts.push_back(O::kAltThreads);
     hipStream_t s = c->s0;
     const int maxsteps = max_chain_steps(c);
-    const bool mixed = c->p.precision == CC_MIXED;
     auto pick = [&](int &slot, const std::vector<double> &t) {
         size_t best = 0;
         for (size_t i = 1; i < t.size(); ++i)
@@ -779,7 +793,7 @@ cc_status tune_threads_n(cc_ctx *c, const double *d_x, std::vector<double> &time
         std::vector<double> t;
         for (int T : Ts) {
             CCCHK(tangent_prologue<N>(c, d_x, s));
-            ccd::TangentParams at = tangent_params<N>(c);
+            ccd::TangentParams<float> at = tangent_params<N, float>(c);
             at.step_begin = 0;
             at.step_end = 8;
             HIPCHK(c, O::tangent(s, T, nt, at));
@@ -794,7 +808,7 @@ cc_status tune_threads_n(cc_ctx *c, const double *d_x, std::vector<double> &time
         }
         pick(c->sched.threads_tangent, t);
         times.insert(times.end(), t.begin(), t.end());
-    } else {
+    } else if (!t64) {
         // every column of the FP64 Jacobian: a slice of steps [4, 4 + Sk)
         const Counts km = counts_of(c->main_plan);
         const int Sk = std::min(maxsteps - 4, 48);
@@ -831,7 +845,7 @@ cc_status timed_step(cc_ctx *c, double *ms)
 
 cc_status tune_stages(cc_ctx *c, std::vector<double> &times)
 {
-    const bool mixed = c->p.precision == CC_MIXED;
+    const bool mixed = tangent_policy(c);
     const bool prof = c->profile;
     c->profile = false;
     cc_schedule best = c->sched;
@@ -887,12 +901,14 @@ cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
     const int N = p->Ntau;
     if (!(N == 128 || N == 256 || N == 512 || N == 1024 || N == 2048)) return CC_BADARG;
     if (p->SchemeIRK != 2 || p->NLeft < 1 || p->NRight < 1 || p->MaxIterIRK < 1) return CC_BADARG;
+    if (p->precision != CC_FP64 && p->precision != CC_MIXED && p->precision != CC_FP64_TANGENT) return CC_BADARG;
     if (!(p->XLeft > 0.0 && p->XLeft < p->XMid && p->XMid < p->XRight && p->XRight < 1.0)) return CC_BADARG;
     cc_ctx *c = new cc_ctx();
     c->p = *p;
     if (c->p.PrecisionIRK32 <= 0.0) c->p.PrecisionIRK32 = 2e-6;
     if (c->p.EpsNewton32 <= 0.0) c->p.EpsNewton32 = 1e-3;
     if (c->p.refine_iters <= 0) c->p.refine_iters = 3;
+    if (p->precision == CC_FP64_TANGENT) c->tangent_rtol = 1e-13;
     c->N = N;
     c->M = N / 2;
     c->n = 3 * N / 4;
@@ -985,10 +1001,17 @@ cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
             ALLOC(*q, mc);
         ALLOC(B->its, mch);
     }
-    ALLOC(c->rec, (size_t)c->X * (size_t)(2 * (5 * (N / 2) + 2 * N + 4)));   // Rec<N>::STEP floats per step
+    {
+        // record and tangent states in the tangent's precision (Rec<N>::STEP values per step)
+        const size_t elem = p->precision == CC_FP64_TANGENT ? sizeof(double) : sizeof(float);
+        char *recp = nullptr, *dyp = nullptr;
+        ALLOC(recp, (size_t)c->X * (size_t)(2 * (5 * (N / 2) + 2 * N + 4)) * elem);
+        ALLOC(dyp, (size_t)n * M * 2 * elem);
+        c->rec = recp;
+        c->dY = dyp;
+    }
     ALLOC(c->rec_base_b, 4);
     ALLOC(c->rec_base_t, n);
-    ALLOC(c->dY, (size_t)n * M);
     for (int32_t **q : {&c->t_start, &c->t_nsteps, &c->t_dir, &c->t_status, &c->t_fail, &c->t_cols, &c->t_lcols,
                         &c->t_rcols, &c->t_lch, &c->t_rch, &c->t_lidx, &c->t_ridx})
         ALLOC(*q, n);
@@ -1150,8 +1173,10 @@ cc_status cc_set_schedule(cc_ctx *c, const cc_schedule *s)
 {
     if (!c || !s) return CC_BADARG;
     if (!valid_threads(c->N, s->threads_base) || !valid_threads(c->N, s->threads_bulk) ||
-        !valid_threads(c->N, s->threads_tangent)) {
-        c->err = "unsupported thread count for this Ntau";
+        !valid_threads(c->N, s->threads_tangent) ||
+        (c->p.precision == CC_FP64_TANGENT && (s->threads_base != ccd::default_threads(c->N) ||
+                                               s->threads_tangent != ccd::default_threads(c->N)))) {
+        c->err = "unsupported thread count for this Ntau and precision";
         return CC_BADARG;
     }
     if (s->steps_per_launch < 16 || s->steps_per_launch > 8192 ||
@@ -1220,11 +1245,11 @@ cc_status cc_jacobian(cc_ctx *c, const double *x, const double *F0, double *J, i
     time_reset(c);
     HIPCHK(c, hipMemcpyAsync(c->x, x, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
     cc_status st;
-    if (c->p.precision == CC_MIXED) {
-        CCCHK(enqueue_mixed(c, c->x));
+    if (tangent_policy(c)) {
+        CCCHK(enqueue_tangent(c, c->x));
         HIPCHK(c, hipMemcpyAsync(J, c->J, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->s0));
         HIPCHK(c, hipStreamSynchronize(c->s0));
-        st = eval_mixed(c, col_status);
+        st = eval_tangent(c, col_status);
     } else {
         // the base column is part of the launch; the difference quotient uses
         // the caller's F0 (NewtonSolver.cpp:684), stored after the last column

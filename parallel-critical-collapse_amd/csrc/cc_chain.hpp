@@ -54,11 +54,13 @@ struct ChainParams {
     double Dim;
     double precision;
     double a00, a01, a10, a11, b0, b1, c0, c1;   // Butcher tableau
-    float *rec;                    // base-trajectory record (tangent-linear columns), may be null
+    void *rec;                     // base-trajectory record (tangent-linear columns), may be null;
+                                   // element type RT of k_chain
     const int64_t *rec_base;       // per chain: first record slot, < 0 = do not record
 };
 
-// Base-trajectory record of one IRK step (FP32): for each stage s, the
+// Base-trajectory record of one IRK step (element type RT of k_chain, the
+// precision of the tangent that reads it): for each stage s, the
 // fields of the last RHS evaluation of the step -- U, V, F, dU/dtau, dV/dtau
 // on j < N/2 (the other half follows by symmetry), r = 1/IA2 and
 // E = e^{F_int} on all N points, and lambda = mean(Coeff1).  A
@@ -81,11 +83,11 @@ struct RhsConst {
 // Evaluate f_s = RHS(yk_s, xs_s) for both stages (ODEStepper.cpp:92-136).
 // bc[0], bc[1]: Re of mode M-2 of each stage state (the mode-2 fix value),
 // published by the caller before the call.
-template <int N, int T, typename R>
+template <int N, int T, typename R, typename RT = float>
 __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> (&f)[2][N / T / 2],
                                          const R (&xs)[2], const RhsConst<R> &K, const R *__restrict__ invm,
                                          cx<R> *xb, R *bc, const cx<R> *__restrict__ tw, int t,
-                                         float *rec = nullptr)
+                                         RT *rec = nullptr)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
     using RC = Rec<N>;
@@ -141,12 +143,12 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
             const R U = half * (a.x - b.x), V = half * (a.y - b.y), F = half * (a.y + b.y);
             const R dU = half * (a2.x - b2.x), dV = half * (a2.y - b2.y);
             if (rec) {
-                float *rs = rec + s * RC::STAGE + t + T * h;
-                rs[RC::U] = (float)U;
-                rs[RC::V] = (float)V;
-                rs[RC::F] = (float)F;
-                rs[RC::DU] = (float)dU;
-                rs[RC::DV] = (float)dV;
+                RT *rs = rec + s * RC::STAGE + t + T * h;
+                rs[RC::U] = (RT)U;
+                rs[RC::V] = (RT)V;
+                rs[RC::F] = (RT)F;
+                rs[RC::DU] = (RT)dU;
+                rs[RC::DV] = (RT)dV;
             }
             const R C1 = -(((X + F) * U * U + (X - F) * V * V) * K.c3 / ((R)8 * X) + K.dm3);
             if (s == 0) c[h].x = C1; else c[h].y = C1;
@@ -200,8 +202,8 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     __syncthreads();
     const R lam0 = bc[2], lam1 = bc[3];
     if (rec && t == 0) {
-        rec[RC::LAM] = (float)lam0;
-        rec[RC::STAGE + RC::LAM] = (float)lam1;
+        rec[RC::LAM] = (RT)lam0;
+        rec[RC::STAGE + RC::LAM] = (RT)lam1;
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -245,9 +247,9 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         for (int s = 0; s < 2; ++s) {
             const R r = E[s][p] / (s ? c[p].y : c[p].x);
             if (rec) {
-                float *rs = rec + s * RC::STAGE + t + T * p;
-                rs[RC::R] = (float)r;
-                rs[RC::E] = (float)E[s][p];
+                RT *rs = rec + s * RC::STAGE + t + T * p;
+                rs[RC::R] = (RT)r;
+                rs[RC::E] = (RT)E[s][p];
             }
             const R cf = CF[s][ph];
             yp[s][p] = {sg * (AU[s][ph] - BU[s][ph] * r), sg * (AV[s][ph] - BV[s][ph] * r) + cf * (r - (R)1)};
@@ -289,7 +291,9 @@ __device__ __forceinline__ RhsConst<R> make_const(double Dim, double delta)
 template <int N, int T, typename R>
 constexpr int chain_waves() { return sizeof(R) == 8 ? (N / T >= 8 ? 1 : 2) : 3; }
 
-template <int N, int T, typename R>
+// RT: element type of the base-trajectory record (float for the FP32 tangent,
+// double for the FP64 one); unused when a.rec is null.
+template <int N, int T, typename R, typename RT = float>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N, T, R>()))) void k_chain(ChainParams<R> a)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
@@ -330,11 +334,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
         __syncthreads();
         bool conv = false;
         int its = 0;
-        float *rec = nullptr;
-        if (a.rec && a.rec_base[ch] >= 0) rec = a.rec + (size_t)(a.rec_base[ch] + step) * Rec<N>::STEP;
+        RT *rec = nullptr;
+        if (a.rec && a.rec_base[ch] >= 0)
+            rec = static_cast<RT *>(a.rec) + (size_t)(a.rec_base[ch] + step) * Rec<N>::STEP;
         for (its = 0; its < a.maxIts; ++its) {
             cx<R> f[2][H];
-            rhs_pair<N, T, R>(yk, f, xs, K, a.invm, xb, bc, a.tw, t, rec);
+            rhs_pair<N, T, R, RT>(yk, f, xs, K, a.invm, xb, bc, a.tw, t, rec);
             R loc = (R)0;
 #pragma unroll
             for (int h = 0; h < H; ++h) {

@@ -31,17 +31,20 @@ struct Ops {
     static hipError_t boundary(hipStream_t s, bool left, bool fp32, int ncols, const double *x, const int32_t *cols,
                                const int32_t *chain, double eps, double Xb, double Dim, const cx<double> *tw, void *Y,
                                double *delta);
-    // k_chain; FP32 chains run at kThreads only
-    static hipError_t chain(hipStream_t s, int T, int nchains, const ChainParams<double> &a);
+    // k_chain; FP32 chains run at kThreads only.  a.rec (if set) is written as
+    // double when rec_fp64 (kThreads only), else as float.
+    static hipError_t chain(hipStream_t s, int T, int nchains, const ChainParams<double> &a, bool rec_fp64 = false);
     static hipError_t chain(hipStream_t s, int T, int nchains, const ChainParams<float> &a);
     static hipError_t mismatch(hipStream_t s, bool fp32, int ncols, const void *Y, const int32_t *lidx,
                                const int32_t *ridx, const double *delta, double precision, const cx<double> *tw,
                                double *F, int32_t *status);
-    static hipError_t boundary_tangent(hipStream_t s, bool left, int ncols, const double *x, const int32_t *cols,
-                                       const int32_t *chain, double hrel, double Xb, double Dim,
-                                       const cx<double> *tw, cx<float> *dY);
-    static hipError_t tangent(hipStream_t s, int T, int nchains, const TangentParams &a);
-    static hipError_t tangent_column(hipStream_t s, int ncols, const cx<float> *dY, const int32_t *cols,
+    // tangent-linear columns; dY is cx<double> when fp64, else cx<float>
+    static hipError_t boundary_tangent(hipStream_t s, bool left, bool fp64, int ncols, const double *x,
+                                       const int32_t *cols, const int32_t *chain, double hrel, double Xb,
+                                       double Dim, const cx<double> *tw, void *dY);
+    static hipError_t tangent(hipStream_t s, int T, int nchains, const TangentParams<float> &a);
+    static hipError_t tangent(hipStream_t s, int T, int nchains, const TangentParams<double> &a);  // kThreads only
+    static hipError_t tangent_column(hipStream_t s, bool fp64, int ncols, const void *dY, const int32_t *cols,
                                      const int32_t *lidx, const int32_t *ridx, const cx<double> *tw, double *J);
     // ICG pack / unpack of a batch of Newton vectors (fields: [batch][3][N], Z: [batch][n])
     static hipError_t pack(hipStream_t s, int batch, const double *fields, const cx<double> *tw, double *Z);

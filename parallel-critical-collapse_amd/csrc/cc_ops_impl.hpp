@@ -33,9 +33,12 @@ hipError_t Ops<N>::boundary(hipStream_t s, bool left, bool fp32, int ncols, cons
 }
 
 template <int N>
-hipError_t Ops<N>::chain(hipStream_t s, int T, int nchains, const ChainParams<double> &a)
+hipError_t Ops<N>::chain(hipStream_t s, int T, int nchains, const ChainParams<double> &a, bool rec_fp64)
 {
-    if (T == kThreads) {
+    if (rec_fp64) {
+        if (T != kThreads) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_chain<N, kThreads, double, double>), dim3((unsigned)nchains), dim3(kThreads), 0, s, a);
+    } else if (T == kThreads) {
         hipLaunchKernelGGL((k_chain<N, kThreads, double>), dim3((unsigned)nchains), dim3(kThreads), 0, s, a);
     } else {
         if constexpr (kAltThreads != 0) {
@@ -73,29 +76,40 @@ hipError_t Ops<N>::mismatch(hipStream_t s, bool fp32, int ncols, const void *Y, 
 }
 
 template <int N>
-hipError_t Ops<N>::boundary_tangent(hipStream_t s, bool left, int ncols, const double *x, const int32_t *cols,
-                                    const int32_t *chain, double hrel, double Xb, double Dim, const cx<double> *tw,
-                                    cx<float> *dY)
+hipError_t Ops<N>::boundary_tangent(hipStream_t s, bool left, bool fp64, int ncols, const double *x,
+                                    const int32_t *cols, const int32_t *chain, double hrel, double Xb, double Dim,
+                                    const cx<double> *tw, void *dY)
 {
     constexpr int T = kThreads;
-    if (left)
-        hipLaunchKernelGGL((k_boundary_tangent<N, T, true>), dim3((unsigned)ncols), dim3(T), 0, s, x, cols, chain,
-                           hrel, Xb, Dim, tw, dY);
-    else
-        hipLaunchKernelGGL((k_boundary_tangent<N, T, false>), dim3((unsigned)ncols), dim3(T), 0, s, x, cols, chain,
-                           hrel, Xb, Dim, tw, dY);
+    const dim3 g((unsigned)ncols), b(T);
+    if (fp64) {
+        if (left)
+            hipLaunchKernelGGL((k_boundary_tangent<N, T, true, double>), g, b, 0, s, x, cols, chain, hrel, Xb, Dim,
+                               tw, static_cast<cx<double> *>(dY));
+        else
+            hipLaunchKernelGGL((k_boundary_tangent<N, T, false, double>), g, b, 0, s, x, cols, chain, hrel, Xb, Dim,
+                               tw, static_cast<cx<double> *>(dY));
+    } else {
+        if (left)
+            hipLaunchKernelGGL((k_boundary_tangent<N, T, true, float>), g, b, 0, s, x, cols, chain, hrel, Xb, Dim,
+                               tw, static_cast<cx<float> *>(dY));
+        else
+            hipLaunchKernelGGL((k_boundary_tangent<N, T, false, float>), g, b, 0, s, x, cols, chain, hrel, Xb, Dim,
+                               tw, static_cast<cx<float> *>(dY));
+    }
     return hipGetLastError();
 }
 
 template <int N>
-hipError_t Ops<N>::tangent(hipStream_t s, int T, int nchains, const TangentParams &a)
+hipError_t Ops<N>::tangent(hipStream_t s, int T, int nchains, const TangentParams<float> &a)
 {
     if (T == kThreads) {
-        hipLaunchKernelGGL((k_tangent<N, kThreads>), dim3((unsigned)nchains), dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((k_tangent<N, kThreads, float>), dim3((unsigned)nchains), dim3(kThreads), 0, s, a);
     } else {
         if constexpr (kAltThreads != 0) {
             if (T != kAltThreads) return hipErrorInvalidValue;
-            hipLaunchKernelGGL((k_tangent<N, kAltThreads>), dim3((unsigned)nchains), dim3(kAltThreads), 0, s, a);
+            hipLaunchKernelGGL((k_tangent<N, kAltThreads, float>), dim3((unsigned)nchains), dim3(kAltThreads), 0, s,
+                               a);
         } else {
             return hipErrorInvalidValue;
         }
@@ -104,11 +118,24 @@ hipError_t Ops<N>::tangent(hipStream_t s, int T, int nchains, const TangentParam
 }
 
 template <int N>
-hipError_t Ops<N>::tangent_column(hipStream_t s, int ncols, const cx<float> *dY, const int32_t *cols,
+hipError_t Ops<N>::tangent(hipStream_t s, int T, int nchains, const TangentParams<double> &a)
+{
+    if (T != kThreads) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_tangent<N, kThreads, double>), dim3((unsigned)nchains), dim3(kThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int N>
+hipError_t Ops<N>::tangent_column(hipStream_t s, bool fp64, int ncols, const void *dY, const int32_t *cols,
                                   const int32_t *lidx, const int32_t *ridx, const cx<double> *tw, double *J)
 {
     constexpr int T = kThreads;
-    hipLaunchKernelGGL((k_tangent_column<N, T>), dim3((unsigned)ncols), dim3(T), 0, s, dY, cols, lidx, ridx, tw, J);
+    if (fp64)
+        hipLaunchKernelGGL((k_tangent_column<N, T, double>), dim3((unsigned)ncols), dim3(T), 0, s,
+                           static_cast<const cx<double> *>(dY), cols, lidx, ridx, tw, J);
+    else
+        hipLaunchKernelGGL((k_tangent_column<N, T, float>), dim3((unsigned)ncols), dim3(T), 0, s,
+                           static_cast<const cx<float> *>(dY), cols, lidx, ridx, tw, J);
     return hipGetLastError();
 }
 

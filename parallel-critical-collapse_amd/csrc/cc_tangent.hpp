@@ -1,4 +1,5 @@
-// cc_tangent.hpp -- tangent-linear Jacobian columns in FP32.
+// cc_tangent.hpp -- tangent-linear Jacobian columns (FP32: the mixed policy's
+// fast path; FP64: an all-FP64 reference Jacobian).
 //
 // The reference builds column i of the Jacobian as the forward difference
 // (F(x + eps e_i) - F(x)) / eps of two full shoots (NewtonSolver.cpp:672-685).
@@ -8,8 +9,8 @@
 //     dK_s = dy + dx sum_j a_sj  Jf(K_j) dK_j,   dy' = dy + dx sum_j b_j Jf(K_j) dK_j,
 // with Jf the linearisation of computeDerivatives (ODEStepper.cpp:92-136) at
 // the base stage fields recorded by the FP64 base shoot (Rec<N> in
-// cc_chain.hpp).  There is no difference of two nearly equal shoots, so FP32
-// keeps its relative precision on every column, also on the weak ones whose
+// cc_chain.hpp, in the precision of the tangent).  There is no difference of
+// two nearly equal shoots, so FP32 keeps its relative precision on every column, also on the weak ones whose
 // finite differences drown in rounding (the Jacobian at these shapes has
 // condition numbers near 1e9).
 //
@@ -28,9 +29,10 @@
 
 namespace ccd {
 
+template <typename R>
 struct TangentParams {
-    cx<float> *dY;            // [nchains][M] tangent states, in/out
-    const float *rec;         // base record
+    cx<R> *dY;            // [nchains][M] tangent states, in/out
+    const R *rec;         // base record (same precision as the tangent)
     const int64_t *rec_base;  // per chain: record slot of its step 0 (the base chain of its side)
     const int32_t *start, *nsteps, *dir;
     const int32_t *base_status;  // status of the base chains (left, right), may be null
@@ -39,44 +41,44 @@ struct TangentParams {
     const double *grid;
     int32_t *status, *fail_idx;
     int64_t *its;
-    const cx<float> *tw;
-    const float *invm;
+    const cx<R> *tw;
+    const R *invm;
     int32_t step_begin, step_end, maxIts;
     double Dim, rtol;         // relative RMS tolerance of the tangent fixed point
     double a00, a01, a10, a11, b0, b1, c0, c1;
 };
 
-template <int N, int T>
-__device__ __forceinline__ void tangent_rhs_pair(const cx<float> (&dk)[2][N / T / 2], cx<float> (&df)[2][N / T / 2],
-                                                 const float *__restrict__ rec, const float (&xs)[2],
-                                                 const RhsConst<float> &K, const float *__restrict__ invm,
-                                                 cx<float> *xb, float *bc, const cx<float> *__restrict__ tw, int t)
+template <int N, int T, typename R>
+__device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2], cx<R> (&df)[2][N / T / 2],
+                                                 const R *__restrict__ rec, const R (&xs)[2],
+                                                 const RhsConst<R> &K, const R *__restrict__ invm,
+                                                 cx<R> *xb, R *bc, const cx<R> *__restrict__ tw, int t)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
     using RC = Rec<N>;
-    const float half = 0.5f, invN = 1.0f / (float)N;
+    const R half = R(0.5), invN = R(1.0) / (R)N;
     asm volatile("" : "+v"(t));
     // per-stage products that survive to the pointwise tangent RHS
-    float dAU[2][H], dBU[2][H], BU[2][H], dAV[2][H], dBV[2][H], BV[2][H], dCF[2][H], CF[2][H];
-    cx<float> c[P];                              // packed h of both stages
+    R dAU[2][H], dBU[2][H], BU[2][H], dAV[2][H], dBV[2][H], BV[2][H], dCF[2][H], CF[2][H];
+    cx<R> c[P];                              // packed h of both stages
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const float *rs = rec + s * RC::STAGE;
-        cx<float> z[2][P];
-        const float fix = bc[s];
+        const R *rs = rec + s * RC::STAGE;
+        cx<R> z[2][P];
+        const R fix = bc[s];
 #pragma unroll
         for (int h = 0; h < H; ++h) {
             const int q = t + T * h;
-            const cx<float> Y = dk[s][h];
-            cx<float> lo, hi;
+            const cx<R> Y = dk[s][h];
+            cx<R> lo, hi;
             if (q < M / 2) {
                 lo = Y;
-                hi = {0.f, 0.f};
+                hi = {R(0.0), R(0.0)};
             } else if (q == M / 2) {
                 lo = cscale(Y, half);
                 hi = Y;
             } else {
-                lo = {0.f, 0.f};
+                lo = {R(0.0), R(0.0)};
                 hi = Y;
             }
             if (q == 2) lo.x = -fix;
@@ -86,30 +88,30 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<float> (&dk)[2][N / T 
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             const int k = t + T * p;
-            const float w = (k == N / 2) ? 0.f : (float)wavenumber<N>(k) * K.k0;
-            const cx<float> v = z[0][p];
+            const R w = (k == N / 2) ? R(0.0) : (R)wavenumber<N>(k) * K.k0;
+            const cx<R> v = z[0][p];
             z[1][p] = {w * v.y, -(w * v.x)};
         }
         fft_block<N, T, -1, 2>(z, xb, tw, t);
-        const float X = xs[s], two_x = 2.f * X, inv_x = 1.f / X;
-        float dC1[H];
+        const R X = xs[s], two_x = R(2.0) * X, inv_x = R(1.0) / X;
+        R dC1[H];
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            const cx<float> a = z[0][h], b = z[0][h + H];
-            const cx<float> a2 = z[1][h], b2 = z[1][h + H];
-            const float dU = half * (a.x - b.x), dV = half * (a.y - b.y), dF = half * (a.y + b.y);
-            const float ddU = half * (a2.x - b2.x), ddV = half * (a2.y - b2.y);
+            const cx<R> a = z[0][h], b = z[0][h + H];
+            const cx<R> a2 = z[1][h], b2 = z[1][h + H];
+            const R dU = half * (a.x - b.x), dV = half * (a.y - b.y), dF = half * (a.y + b.y);
+            const R ddU = half * (a2.x - b2.x), ddV = half * (a2.y - b2.y);
             const int j = t + T * h;
-            const float U = rs[RC::U + j], V = rs[RC::V + j], F = rs[RC::F + j];
-            const float tU = rs[RC::DU + j], tV = rs[RC::DV + j];
+            const R U = rs[RC::U + j], V = rs[RC::V + j], F = rs[RC::F + j];
+            const R tU = rs[RC::DU + j], tV = rs[RC::DV + j];
             // d Coeff1 (ICG.cpp:834-835 differentiated)
-            dC1[h] = -(K.c3 / (8.f * X)) *
-                     (dF * U * U + 2.f * (X + F) * U * dU - dF * V * V + 2.f * (X - F) * V * dV);
-            const float inv1 = 1.f / (two_x * (F + X)), inv2 = 1.f / (two_x * (F - X));
-            const float dinv1 = -inv1 * inv1 * two_x * dF, dinv2 = -inv2 * inv2 * two_x * dF;
-            const float S = U + V, dS = dU + dV;
-            const float numU = F * K.dm2 * S - two_x * tU, numV = F * K.dm2 * S + two_x * tV;
-            const float dnum = K.dm2 * (dF * S + F * dS);
+            dC1[h] = -(K.c3 / (R(8.0) * X)) *
+                     (dF * U * U + R(2.0) * (X + F) * U * dU - dF * V * V + R(2.0) * (X - F) * V * dV);
+            const R inv1 = R(1.0) / (two_x * (F + X)), inv2 = R(1.0) / (two_x * (F - X));
+            const R dinv1 = -inv1 * inv1 * two_x * dF, dinv2 = -inv2 * inv2 * two_x * dF;
+            const R S = U + V, dS = dU + dV;
+            const R numU = F * K.dm2 * S - two_x * tU, numV = F * K.dm2 * S + two_x * tV;
+            const R dnum = K.dm2 * (dF * S + F * dS);
             dAU[s][h] = (dnum - two_x * ddU) * inv1 + numU * dinv1;
             BU[s][h] = K.two_dm3 * F * U * inv1;
             dBU[s][h] = K.two_dm3 * ((dF * U + F * dU) * inv1 + F * U * dinv1);
@@ -124,7 +126,7 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<float> (&dk)[2][N / T 
         for (int p = 0; p < P; ++p) {
             const int ph = (p < H) ? p : p - H;
             const int j = t + T * p;
-            const float hv = -dC1[ph] * rs[RC::E + j] / rs[RC::R + j];
+            const R hv = -dC1[ph] * rs[RC::E + j] / rs[RC::R + j];
             if (s == 0) c[p].x = hv; else c[p].y = hv;
         }
     }
@@ -135,32 +137,32 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<float> (&dk)[2][N / T 
 #pragma unroll
     for (int p = 0; p < P; ++p) xb[t + T * p] = c[p];
     __syncthreads();
-    cx<float> cm[P];
+    cx<R> cm[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) cm[p] = xb[(N - (t + T * p)) & (N - 1)];
     __syncthreads();
-    const float lam0 = rec[RC::LAM], lam1 = rec[RC::STAGE + RC::LAM];
+    const R lam0 = rec[RC::LAM], lam1 = rec[RC::STAGE + RC::LAM];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         const int k = t + T * p;
-        const cx<float> A = c[p];
-        const cx<float> B = {cm[p].x, -cm[p].y};
-        const cx<float> S0 = cscale(A + B, half);
-        const cx<float> Dd = A - B;
-        const cx<float> S1 = {half * Dd.y, -(half * Dd.x)};
-        cx<float> G[2];
+        const cx<R> A = c[p];
+        const cx<R> B = {cm[p].x, -cm[p].y};
+        const cx<R> S0 = cscale(A + B, half);
+        const cx<R> Dd = A - B;
+        const cx<R> S1 = {half * Dd.y, -(half * Dd.x)};
+        cx<R> G[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const cx<float> hs = s ? S1 : S0;
-            const float lam = s ? lam1 : lam0;
+            const cx<R> hs = s ? S1 : S0;
+            const R lam = s ? lam1 : lam0;
             if (k == 0) {
-                G[s] = (lam != 0.f) ? cx<float>{hs.x / lam, hs.y / lam} : cx<float>{0.f, 0.f};
+                G[s] = (lam != R(0.0)) ? cx<R>{hs.x / lam, hs.y / lam} : cx<R>{R(0.0), R(0.0)};
             } else if (k == N / 2) {
                 G[s] = cscale(hs, lam / (lam * lam + K.q2));
             } else {
-                const float w = (float)wavenumber<N>(k) * K.k0;
-                const float den = lam * lam + w * w;
-                const float inv = 1.f / den;
+                const R w = (R)wavenumber<N>(k) * K.k0;
+                const R den = lam * lam + w * w;
+                const R inv = R(1.0) / den;
                 G[s] = {(hs.x * lam - hs.y * w) * inv, (hs.y * lam + hs.x * w) * inv};
             }
         }
@@ -168,21 +170,21 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<float> (&dk)[2][N / T 
     }
     fft1<N, T, -1>(c, xb, tw, t);
     // pointwise tangent RHS
-    cx<float> yp[2][P];
+    cx<R> yp[2][P];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         const int ph = (p < H) ? p : p - H;
-        const float sg = (p < H) ? 1.f : -1.f;
+        const R sg = (p < H) ? R(1.0) : -R(1.0);
         const int j = t + T * p;
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const float *rs = rec + s * RC::STAGE;
-            const float r = rs[RC::R + j], E = rs[RC::E + j];
-            const float dIA2 = (s ? c[p].y : c[p].x) / E;
-            const float dr = -r * r * dIA2;
-            const float dUx = sg * (dAU[s][ph] - dBU[s][ph] * r - BU[s][ph] * dr);
-            const float dVx = sg * (dAV[s][ph] - dBV[s][ph] * r - BV[s][ph] * dr);
-            const float dFx = dCF[s][ph] * (r - 1.f) + CF[s][ph] * dr;
+            const R *rs = rec + s * RC::STAGE;
+            const R r = rs[RC::R + j], E = rs[RC::E + j];
+            const R dIA2 = (s ? c[p].y : c[p].x) / E;
+            const R dr = -r * r * dIA2;
+            const R dUx = sg * (dAU[s][ph] - dBU[s][ph] * r - BU[s][ph] * dr);
+            const R dVx = sg * (dAV[s][ph] - dBV[s][ph] * r - BV[s][ph] * dr);
+            const R dFx = dCF[s][ph] * (r - R(1.0)) + CF[s][ph] * dr;
             yp[s][p] = {dUx, dVx + dFx};
         }
     }
@@ -192,24 +194,28 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<float> (&dk)[2][N / T 
 #pragma unroll
         for (int h = 0; h < H; ++h) {
             const int q = t + T * h;
-            const cx<float> lo = cscale(yp[s][h], invN), hi = cscale(yp[s][h + H], invN);
-            cx<float> v = (q < M / 2) ? lo : ((q == M / 2) ? lo + hi : hi);
-            if (q == 2) v.x = 0.f;
+            const cx<R> lo = cscale(yp[s][h], invN), hi = cscale(yp[s][h + H], invN);
+            cx<R> v = (q < M / 2) ? lo : ((q == M / 2) ? lo + hi : hi);
+            if (q == 2) v.x = R(0.0);
             df[s][h] = v;
         }
 }
 
-// Register budget: 3 waves per SIMD (168 VGPRs) at four values per thread,
-// 2 (256 VGPRs) at eight, so that a launch of n two-wave workgroups still
-// fits the chip in one round.
-template <int N, int T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(N / T >= 8 ? 2 : 3))) void k_tangent(TangentParams a)
+// Register budget of the FP32 tangent: 3 waves per SIMD (168 VGPRs) at four
+// values per thread, 2 (256 VGPRs) at eight, so that a launch of n two-wave
+// workgroups still fits the chip in one round.  The FP64 tangent (an
+// all-FP64 reference Jacobian, not the fast path) gets 2 waves at four values.
+template <int N, int T, typename R>
+constexpr int tangent_waves() { return sizeof(R) == 8 ? (N / T >= 8 ? 1 : 2) : (N / T >= 8 ? 2 : 3); }
+
+template <int N, int T, typename R>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves<N, T, R>()))) void k_tangent(TangentParams<R> a)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
-    __shared__ cx<float> xb[2 * N];
-    __shared__ cx<float> ys[2][M];
-    __shared__ float bc[8];
-    __shared__ float red[(T / 64) > 0 ? T / 64 : 1];
+    __shared__ cx<R> xb[2 * N];
+    __shared__ cx<R> ys[2][M];
+    __shared__ R bc[8];
+    __shared__ R red[(T / 64) > 0 ? T / 64 : 1];
     const int t = threadIdx.x;
     const int ch = blockIdx.x;
     if (a.status[ch] != 0) return;
@@ -220,22 +226,22 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(N / T >= 8 ? 
     const int s_begin = a.step_begin;
     const int s_end = min(a.step_end, nst);
     if (s_begin >= s_end) return;
-    cx<float> *Yg = a.dY + (size_t)ch * M;
+    cx<R> *Yg = a.dY + (size_t)ch * M;
 #pragma unroll
     for (int h = 0; h < H; ++h) ys[0][t + T * h] = Yg[t + T * h];
-    const RhsConst<float> K = make_const<N, float>(a.Dim, a.delta[0]);
+    const RhsConst<R> K = make_const<N, R>(a.Dim, a.delta[0]);
     int idx = a.start[ch] + dir * s_begin;
     int64_t its_sum = 0;
     int32_t st = 0;
-    cx<float> dk[2][H];
+    cx<R> dk[2][H];
     for (int step = s_begin; step < s_end; ++step, idx += dir) {
         const double xin = a.grid[idx], xout = a.grid[idx + dir];
         const double dxd = xout - xin;
-        const float xs[2] = {(float)(xin + dxd * a.c0), (float)(xin + dxd * a.c1)};
-        const float A00 = (float)(dxd * a.a00), A01 = (float)(dxd * a.a01), A10 = (float)(dxd * a.a10),
-                    A11 = (float)(dxd * a.a11);
-        const float B0 = (float)(dxd * a.b0), B1 = (float)(dxd * a.b1);
-        const float *rec = a.rec + (size_t)(a.rec_base[ch] + step) * Rec<N>::STEP;
+        const R xs[2] = {(R)(xin + dxd * a.c0), (R)(xin + dxd * a.c1)};
+        const R A00 = (R)(dxd * a.a00), A01 = (R)(dxd * a.a01), A10 = (R)(dxd * a.a10),
+                    A11 = (R)(dxd * a.a11);
+        const R B0 = (R)(dxd * a.b0), B1 = (R)(dxd * a.b1);
+        const R *rec = a.rec + (size_t)(a.rec_base[ch] + step) * Rec<N>::STEP;
 #pragma unroll
         for (int h = 0; h < H; ++h) dk[0][h] = dk[1][h] = ys[0][t + T * h];
         if (t == T - 2) bc[0] = bc[1] = dk[0][H - 1].x;
@@ -243,21 +249,21 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(N / T >= 8 ? 
         bool conv = false;
         int its = 0;
         for (its = 0; its < a.maxIts; ++its) {
-            cx<float> df[2][H];
-            tangent_rhs_pair<N, T>(dk, df, rec, xs, K, a.invm, xb, bc, a.tw, t);
-            float loc = 0.f, mag = 0.f;
+            cx<R> df[2][H];
+            tangent_rhs_pair<N, T, R>(dk, df, rec, xs, K, a.invm, xb, bc, a.tw, t);
+            R loc = R(0.0), mag = R(0.0);
 #pragma unroll
             for (int h = 0; h < H; ++h) {
-                const cx<float> y = ys[0][t + T * h];
-                cx<float> n0 = y, n1 = y, nn = y;
+                const cx<R> y = ys[0][t + T * h];
+                cx<R> n0 = y, n1 = y, nn = y;
                 n0.x += A00 * df[0][h].x; n0.y += A00 * df[0][h].y;
                 n0.x += A01 * df[1][h].x; n0.y += A01 * df[1][h].y;
                 n1.x += A10 * df[0][h].x; n1.y += A10 * df[0][h].y;
                 n1.x += A11 * df[1][h].x; n1.y += A11 * df[1][h].y;
                 nn.x += B0 * df[0][h].x; nn.y += B0 * df[0][h].y;
                 nn.x += B1 * df[1][h].x; nn.y += B1 * df[1][h].y;
-                const float d0x = dk[0][h].x - n0.x, d0y = dk[0][h].y - n0.y;
-                const float d1x = dk[1][h].x - n1.x, d1y = dk[1][h].y - n1.y;
+                const R d0x = dk[0][h].x - n0.x, d0y = dk[0][h].y - n0.y;
+                const R d1x = dk[1][h].x - n1.x, d1y = dk[1][h].y - n1.y;
                 loc += d0x * d0x + d0y * d0y + d1x * d1x + d1y * d1y;
                 mag += n0.x * n0.x + n0.y * n0.y + n1.x * n1.x + n1.y * n1.y;
                 dk[0][h] = n0;
@@ -268,11 +274,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(N / T >= 8 ? 
                 bc[0] = dk[0][H - 1].x;
                 bc[1] = dk[1][H - 1].x;
             }
-            const float d2 = block_sum<T, float>(loc, red, t);
-            const float m2 = block_sum<T, float>(mag, red, t);
+            const R d2 = block_sum<T, R>(loc, red, t);
+            const R m2 = block_sum<T, R>(mag, red, t);
             __syncthreads();
             // linear fixed point: stop when the update is below rtol of the iterate
-            if (d2 <= (float)(a.rtol * a.rtol) * m2) {
+            if (d2 <= (R)(a.rtol * a.rtol) * m2) {
                 conv = true;
                 break;
             }
@@ -296,10 +302,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(N / T >= 8 ? 
 
 // Tangent boundary states of column cols[b]: central difference of the FP64
 // boundary series, d Y0 / d x_i = (Y0(x + h e_i) - Y0(x - h e_i)) / 2h.
-template <int N, int T, bool LEFT>
+template <int N, int T, bool LEFT, typename R>
 __global__ __launch_bounds__(T) void k_boundary_tangent(const double *x, const int32_t *cols, const int32_t *chain,
                                                          double hrel, double Xb, double Dim, const cx<double> *tw,
-                                                         cx<float> *dY)
+                                                         cx<R> *dY)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
     __shared__ cx<double> xb[N];
@@ -330,15 +336,15 @@ __global__ __launch_bounds__(T) void k_boundary_tangent(const double *x, const i
     for (int q = 0; q < H; ++q) {
         cx<double> d = {(Yp[q].x - Ym[q].x) * inv2h, (Yp[q].y - Ym[q].y) * inv2h};
         if (t == 2 && q == 0) d.x = 0.0;          // Re Y[2] carries Delta, fixed here
-        dY[(size_t)ch * M + t + T * q] = {(float)d.x, (float)d.y};
+        dY[(size_t)ch * M + t + T * q] = {(R)d.x, (R)d.y};
     }
 }
 
 // Jacobian column from the tangent state at the match point: the mismatch,
 // state -> fields and pack are linear (ShootingSolver.cpp:154-182,
 // ICG.cpp:891-932, 678-711).  lidx/ridx < 0: that side has a zero tangent.
-template <int N, int T>
-__global__ __launch_bounds__(T) void k_tangent_column(const cx<float> *dY, const int32_t *cols, const int32_t *lidx,
+template <int N, int T, typename R>
+__global__ __launch_bounds__(T) void k_tangent_column(const cx<R> *dY, const int32_t *cols, const int32_t *lidx,
                                                        const int32_t *ridx, const cx<double> *tw, double *J)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2, n = 3 * N / 4;

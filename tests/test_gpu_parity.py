@@ -288,3 +288,51 @@ def test_mixed_newton_step_solves_fp64_refined_system(c1_reference):
     assert np.max(np.abs(F0 - F0_ref)) < 1e-9
     J, _, _ = ctx.jacobian(s["x"], F0)
     assert np.linalg.norm(J @ dx + F0) / np.linalg.norm(F0) < 1e-8
+
+
+@pytest.fixture(scope="module")
+def c1_tangent_jacobians(c1_reference):
+    s, F0, _, _, _ = c1_reference
+    out = {}
+    for pol in (cc.CC_FP64_TANGENT, cc.CC_MIXED):
+        print("tangent Jacobian, precision", pol, flush=True)
+        J, cs, st = ctx_for(s, precision=pol).jacobian(s["x"], F0)
+        assert st == 0 and not cs.any()
+        out[pol] = J
+    return out
+
+
+def test_fp64_tangent_jacobian_matches_oracle_fd(c1_reference, c1_tangent_jacobians):
+    """CC_FP64_TANGENT: FP64 tangent-linear columns against the oracle's FP64
+    forward differences (whose own error, ~3e-8 absolute per entry, dominates
+    the comparison: bounds as for the FP32 tangent, median tighter)."""
+    s, F0, J_ref, _, _ = c1_reference
+    J = c1_tangent_jacobians[cc.CC_FP64_TANGENT]
+    dslot = s["shape"].Ntau // 2 + 2
+    assert rel(J[:, dslot], J_ref[:, dslot]) < 1e-5
+    cn = np.maximum(np.linalg.norm(J_ref, axis=0), 1e-4)
+    err = np.linalg.norm(J - J_ref, axis=0) / cn
+    print("FP64 tangent vs FD oracle: max", err.max(), "median", np.median(err))
+    assert np.median(err) < 1e-4 and err.max() < 1e-2
+    assert rel(J, J_ref) < 1e-4
+
+
+def test_fp32_tangent_against_fp64_tangent(c1_tangent_jacobians):
+    """The FP32 tangent columns against the FP64 tangent columns of the same
+    linearisation: the error of the mixed policy's Jacobian proper, relative
+    to each column's own norm (no floor)."""
+    J64 = c1_tangent_jacobians[cc.CC_FP64_TANGENT]
+    J32 = c1_tangent_jacobians[cc.CC_MIXED]
+    err = np.linalg.norm(J32 - J64, axis=0) / np.linalg.norm(J64, axis=0)
+    print("FP32 vs FP64 tangent columns: max", err.max(), "median", np.median(err))
+    assert np.median(err) < 1e-5 and err.max() < 1e-3
+    assert rel(J32, J64) < 1e-5
+
+
+def test_fp64_tangent_newton_step(c1_reference):
+    s, F0_ref, _, _, _ = c1_reference
+    ctx = ctx_for(s, precision=cc.CC_FP64_TANGENT)
+    F0, dx = ctx.newton_step(s["x"])
+    assert np.max(np.abs(F0 - F0_ref)) < 1e-9
+    J, _, _ = ctx.jacobian(s["x"], F0)
+    assert np.linalg.norm(J @ dx + F0) / np.linalg.norm(F0) < 1e-9
