@@ -30,6 +30,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
+# HBM traffic per launch of the dominant kernel, from two rocprofv3 --pmc
+# passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a plain
+# device copy; see profiles/hbm_traffic_newton_step_ntau1024_x8000.md.  Keyed
+# by the kernel the bench reports; a kernel without a measurement reports null.
+MEASURED_TRAFFIC_BYTES = {
+    "k_tangent<1024,256>": (98.1e6 + 3.2e6),
+}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -242,7 +251,8 @@ def main():
     peak = 157.3
     roof = {"bound": "mfma", "pipe": "fp32 valu (FFT + pointwise; no dense contraction)",
             "kernel": f"k_tangent<{N},{T}>", "achieved": round(achieved, 3), "peak": peak,
-            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+            "traffic": MEASURED_TRAFFIC_BYTES.get(f"k_tangent<{N},{T}>"),
             "flops_per_step": fl / args.steps, "kernel_s_per_step": mixed["chain_ms"] / 1e3 / args.steps,
             "avg_launch_ms": mixed["chain_ms"] / max(mixed["launches"], 1),
             "launches_per_step": mixed["launches"] / args.steps}

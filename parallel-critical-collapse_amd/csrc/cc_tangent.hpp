@@ -58,6 +58,11 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
     using RC = Rec<N>;
     const R half = R(0.5), invN = R(1.0) / (R)N;
     asm volatile("" : "+v"(t));
+    // The record of a step is the same for every fixed-point iteration; an
+    // opaque pointer keeps its loads (and everything computed from them) in
+    // the iteration instead of hoisted in front of the loop, where they would
+    // spill.
+    asm volatile("" : "+s"(rec));
     // per-stage products that survive to the pointwise tangent RHS
     R dAU[2][H], dBU[2][H], BU[2][H], dAV[2][H], dBV[2][H], BV[2][H], dCF[2][H], CF[2][H];
     cx<R> c[P];                              // packed h of both stages
@@ -93,6 +98,8 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             z[1][p] = {w * v.y, -(w * v.x)};
         }
         fft_block<N, T, -1, 2>(z, xb, tw, t);
+        // the record loads of this stage stay behind its transform
+        asm volatile("" : "+s"(rs));
         const R X = xs[s], two_x = R(2.0) * X, inv_x = R(1.0) / X;
         R dC1[H];
 #pragma unroll
