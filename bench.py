@@ -139,7 +139,7 @@ def rms(v):
     return float(np.sqrt(np.mean(np.asarray(v) ** 2)))
 
 
-def newton_study(cc, shape, policies, tol, max_its, tune):
+def newton_study(cc, shape, policies, tol, max_its, tune, save_failures=None):
     """Newton from the seeded input to mismatch < tol under each precision
     policy, with the reference's damping fac = min(1, SlowError/err) and its
     divergence guard (NewtonSolver.cpp:298-358): iterations, time per
@@ -155,10 +155,17 @@ def newton_study(cc, shape, policies, tol, max_its, tune):
         if tune:
             ctx.autotune(x)
         hist, secs = [], []
-        err, err_old, increase, reached = 1.0, 1.0, 0, None
+        err, err_old, increase, reached, failure = 1.0, 1.0, 0, None, None
         for its in range(max_its):
             t = time.perf_counter()
-            F0, dx = ctx.newton_step(x)
+            try:
+                F0, dx = ctx.newton_step(x)
+            except cc.CCError as e:
+                failure = {"iteration": its + 1, "error": str(e)}
+                log(f"[newton] {name} iteration {its + 1}: {e}")
+                if save_failures:
+                    np.save(os.path.join(save_failures, f"newton_fail_{name}_N{shape.Ntau}.npy"), x)
+                break
             err_old, err = err, rms(F0)
             hist.append(err)
             log(f"[newton] {name} iteration {its + 1}: mismatch {err:.3e}")
@@ -172,9 +179,10 @@ def newton_study(cc, shape, policies, tol, max_its, tune):
                 break
             x = x + min(1.0, shape.SlowError / err) * dx
             secs.append(time.perf_counter() - t)
-        out[name] = {"iterations_to_tol": reached, "mismatch": hist, "final_mismatch": hist[-1],
-                     "Delta": float(x[dslot]), "s_per_iteration": float(np.mean(secs)),
-                     "total_s": float(np.sum(secs)), "schedule": ctx.schedule().as_dict()}
+        out[name] = {"iterations_to_tol": reached, "mismatch": hist, "final_mismatch": hist[-1] if hist else None,
+                     "Delta": float(x[dslot]), "s_per_iteration": float(np.mean(secs)) if secs else None,
+                     "s_iterations": [round(v, 4) for v in secs], "total_s": float(np.sum(secs)),
+                     "failure": failure, "schedule": ctx.schedule().as_dict()}
         ctx.close()
     ref = "fp64_tangent" if "fp64_tangent" in out else None
     if ref:
@@ -213,6 +221,8 @@ def main():
     ap.add_argument("--policies", default="mixed,fp64_tangent,fp64")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-its", type=int, default=12)
+    ap.add_argument("--save-failures", default="", metavar="DIR",
+                    help="--newton: save the Newton vector at which a policy failed to DIR")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -223,7 +233,8 @@ def main():
         cc = importlib.import_module("parallel-critical-collapse_amd")
         W = importlib.import_module("parallel-critical-collapse_amd.workload")
         shape = W.config(args.config)
-        res = newton_study(cc, shape, args.policies.split(","), args.tol, args.max_its, not args.no_tune)
+        res = newton_study(cc, shape, args.policies.split(","), args.tol, args.max_its, not args.no_tune,
+                           args.save_failures or None)
         print(json.dumps({"workload": f"newton Ntau={shape.Ntau} x_points={shape.X} n={shape.n} Dim={shape.Dim} "
                                       f"XMid={shape.XMid} scale={shape.scale}", "tol": args.tol,
                           "policies": res}), flush=True)

@@ -150,6 +150,32 @@ enum { CC_TUNE_THREADS = 1, CC_TUNE_STAGES = 2 };
 cc_status cc_autotune(cc_ctx *ctx, const double *x, int32_t flags, cc_schedule *out, double *times,
                       int32_t ntimes, int32_t *ntried);
 
+/* ---- batched independent problems ---------------------------------------- */
+
+/* A set of independent problems (any mix of Ntau, grid sizes and Dim; one
+ * tangent precision policy, MaxIterIRK, PrecisionIRK, EpsNewton for all),
+ * stepped together: per Newton step, the base chains of all problems of one
+ * Ntau run in one launch per chunk and their tangent chains in another,
+ * problems of different Ntau concurrently.  The batch form of the loop body
+ * of NewtonSolver::run (NewtonSolver.cpp:298-358) over a dimension sweep
+ * (main.cpp:236-356 runs those one after another). */
+typedef struct cc_batch cc_batch;
+cc_status cc_batch_create(const cc_problem *problems, int32_t count, cc_batch **out);
+cc_status cc_batch_destroy(cc_batch *b);
+const char *cc_batch_last_error(const cc_batch *b);
+cc_status cc_batch_info(cc_batch *b, int32_t *nproblems, int32_t *nclasses);
+/* x, F0, dx: `count` host pointers (n_i doubles each; F0/dx entries may be
+ * NULL); status[count] receives each problem's cc_status.  Returns the worst. */
+cc_status cc_batch_newton_step(cc_batch *b, const double *const *x, double *const *F0, double *const *dx,
+                               int32_t *status);
+/* Schedules per Ntau class (cls < nclasses, ascending Ntau); use_graph is
+ * batch-wide. */
+cc_status cc_batch_get_schedule(cc_batch *b, int32_t cls, int32_t *Ntau, cc_schedule *out);
+cc_status cc_batch_set_schedule(cc_batch *b, int32_t cls, const cc_schedule *s);
+cc_status cc_batch_autotune(cc_batch *b, const double *const *x, double *times, int32_t ntimes, int32_t *ntried);
+/* Summed counters of the last batch step (as cc_counters). */
+cc_status cc_batch_counters(cc_batch *b, int64_t out[3]);
+
 /* ---- unit entry points (the reference's lower-level interfaces) ---------- */
 
 /* SpectralTransformer forwardFFT / backwardFFT, complex-to-complex

@@ -119,11 +119,21 @@ def load():
     L.cc_chain_timing.argtypes = [_vp, _dp, _ip, _lp, _lp]
     L.cc_pack.argtypes = [_vp, _dp, _dp, C.c_int32]
     L.cc_unpack.argtypes = [_vp, _dp, _dp, C.c_int32]
+    _pp = C.POINTER(_dp)
+    L.cc_batch_create.argtypes = [C.POINTER(Problem), C.c_int32, C.POINTER(_vp)]
+    L.cc_batch_destroy.argtypes = [_vp]
+    L.cc_batch_last_error.argtypes = [_vp]
+    L.cc_batch_info.argtypes = [_vp, _ip, _ip]
+    L.cc_batch_newton_step.argtypes = [_vp, _pp, _pp, _pp, _ip]
+    L.cc_batch_get_schedule.argtypes = [_vp, C.c_int32, _ip, C.POINTER(Schedule)]
+    L.cc_batch_set_schedule.argtypes = [_vp, C.c_int32, C.POINTER(Schedule)]
+    L.cc_batch_autotune.argtypes = [_vp, _pp, _dp, C.c_int32, _ip]
+    L.cc_batch_counters.argtypes = [_vp, _lp]
     L.cc_get_schedule.argtypes = [_vp, C.POINTER(Schedule)]
     L.cc_set_schedule.argtypes = [_vp, C.POINTER(Schedule)]
     L.cc_autotune.argtypes = [_vp, _dp, C.c_int32, C.POINTER(Schedule), _dp, C.c_int32, _ip]
     for name in declared_symbols():
-        getattr(L, name).restype = C.c_char_p if name == "cc_last_error" else C.c_int
+        getattr(L, name).restype = C.c_char_p if name.endswith("last_error") else C.c_int
     _lib = L
     return L
 
@@ -297,3 +307,77 @@ class Context:
         self._check(load().cc_expansions(self.h, _d(x), YL.ctypes.data_as(_dp), YR.ctypes.data_as(_dp)),
                     "cc_expansions")
         return YL, YR
+
+
+def _ptrs(arrays):
+    return (_dp * len(arrays))(*[a.ctypes.data_as(_dp) for a in arrays])
+
+
+class Batch:
+    """A cc_batch: independent problems stepped together (one launch per Ntau
+    class and chunk)."""
+
+    def __init__(self, problems):
+        L = load()
+        self.problems = list(problems)
+        arr = (Problem * len(self.problems))(*self.problems)
+        h = _vp()
+        st = L.cc_batch_create(arr, len(self.problems), C.byref(h))
+        if st != CC_OK:
+            raise CCError(st, "cc_batch_create failed")
+        self.h = h
+        self.n = [3 * p.Ntau // 4 for p in self.problems]
+        np_, nc = C.c_int32(0), C.c_int32(0)
+        L.cc_batch_info(self.h, C.byref(np_), C.byref(nc))
+        self.nclasses = nc.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().cc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != CC_OK:
+            raise CCError(st, f"{what}: {load().cc_batch_last_error(self.h).decode()}")
+
+    def newton_step(self, xs, check=True):
+        """xs: list of Newton vectors -> (F0 list, dx list, per-problem status, worst status)."""
+        xs = [_arr(x) for x in xs]
+        F0 = [np.zeros(n) for n in self.n]
+        dx = [np.zeros(n) for n in self.n]
+        status = np.zeros(len(xs), np.int32)
+        st = load().cc_batch_newton_step(self.h, _ptrs(xs), _ptrs(F0), _ptrs(dx), status.ctypes.data_as(_ip))
+        if check:
+            self._check(st, "cc_batch_newton_step")
+        return F0, dx, status, st
+
+    def schedule(self, cls):
+        sc, N = Schedule(), C.c_int32(0)
+        self._check(load().cc_batch_get_schedule(self.h, cls, C.byref(N), C.byref(sc)), "cc_batch_get_schedule")
+        return N.value, sc
+
+    def set_schedule(self, cls, **kw):
+        _, sc = self.schedule(cls)
+        for k, v in kw.items():
+            setattr(sc, k, int(v))
+        self._check(load().cc_batch_set_schedule(self.h, cls, C.byref(sc)), "cc_batch_set_schedule")
+        return sc
+
+    def autotune(self, xs):
+        xs = [_arr(x) for x in xs]
+        times = np.zeros(128)
+        nt = C.c_int32(0)
+        self._check(load().cc_batch_autotune(self.h, _ptrs(xs), _d(times), times.size, C.byref(nt)),
+                    "cc_batch_autotune")
+        return times[:nt.value].copy()
+
+    def counters(self):
+        out = np.zeros(3, np.int64)
+        load().cc_batch_counters(self.h, out.ctypes.data_as(_lp))
+        return out

@@ -110,7 +110,7 @@ struct cc_ctx {
     int64_t *t_its = nullptr;
     std::vector<int32_t> t_cols_h, t_nsteps_h;
     int t_nl = 0, t_nr = 0;
-    double tangent_rtol = 1e-6, tangent_h = 1e-5;   // CC_FP64_TANGENT: rtol 1e-13
+    double tangent_rtol = 1e-6, tangent_h = 1e-5;   // CC_FP64_TANGENT: rtol 1e-13 (1e-12 at Ntau 2048)
     int maxcols = 0, maxchains = 0;
     cc_schedule sched{};
     void *hblock = nullptr;
@@ -127,6 +127,44 @@ struct cc_ctx {
     // captured Newton step
     hipGraphExec_t gexec = nullptr;
     const void *gkey[3] = {nullptr, nullptr, nullptr};
+};
+
+// Problems of one Ntau inside a batch: their base and tangent chains run in
+// shared launches; per-chain D, grid offset and base index tell them apart.
+// Member contexts keep their own plans, Jacobian and LU buffers; their chain
+// arrays, record and tangent states point into the class arrays below.
+struct BatchClass {
+    int N = 0;
+    std::vector<int> members;
+    hipStream_t s0 = nullptr, s1 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    std::vector<hipEvent_t> chunk_ev;
+    cc_schedule sched{};
+    int nbase = 0, ntan = 0, maxsteps = 0;
+    // base launch: 4 chains per member (base L, base R, Delta L, Delta R)
+    cx<double> *Y = nullptr;
+    double *delta = nullptr, *dim = nullptr, *grid = nullptr;
+    int32_t *start = nullptr, *nsteps = nullptr, *dir = nullptr, *status = nullptr, *fail = nullptr, *goff = nullptr;
+    int64_t *its = nullptr, *rec_base = nullptr;
+    // tangent launch
+    void *dY = nullptr, *rec = nullptr;
+    double *t_dim = nullptr;
+    int32_t *t_start = nullptr, *t_nsteps = nullptr, *t_dir = nullptr, *t_status = nullptr, *t_fail = nullptr,
+            *t_goff = nullptr, *t_bs = nullptr;
+    int64_t *t_its = nullptr, *t_rec_base = nullptr;
+};
+
+struct cc_batch {
+    std::vector<cc_ctx *> ctx;
+    std::vector<BatchClass> classes;
+    std::vector<void *> allocs;
+    std::string err;
+    bool use_graph = false;
+    hipStream_t s_main = nullptr;        // origin of the batch step: forks to every class
+    hipEvent_t ev_start = nullptr;
+    std::vector<hipEvent_t> ev_done;
+    hipGraphExec_t gexec = nullptr;
+    int64_t counters[3] = {0, 0, 0};
 };
 
 namespace {
@@ -887,6 +925,218 @@ cc_status tune_stages(cc_ctx *c, std::vector<double> &times)
     return CC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batched independent problems (cc_batch_*)
+// ---------------------------------------------------------------------------
+bool bfail(cc_batch *b, const char *what, hipError_t e)
+{
+    if (e == hipSuccess) return false;
+    b->err = std::string(what) + ": " + hipGetErrorString(e);
+    return true;
+}
+
+#define BCHK(b, call)                                                  \
+    do {                                                               \
+        hipError_t e_ = (call);                                        \
+        if (bfail((b), #call, e_)) return CC_DEVICE;                   \
+    } while (0)
+
+// Propagates a member's error into the batch.
+cc_status bstat(cc_batch *b, cc_ctx *c, cc_status st)
+{
+    if (st != CC_OK) b->err = c->err;
+    return st;
+}
+
+template <typename Tp>
+hipError_t balloc(cc_batch *b, Tp **p, size_t count)
+{
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(Tp));
+    if (e == hipSuccess) {
+        b->allocs.push_back(q);
+        *p = static_cast<Tp *>(q);
+    }
+    return e;
+}
+
+// One Newton step of every member of class C, on C.s0 (C.s1 joins when
+// pipelined): per-member boundary data, shared base and tangent launches in
+// chunks, per-member mismatch / Jacobian columns / solve.
+template <int N, typename R>
+cc_status batch_enqueue_class_n(cc_batch *b, BatchClass &C)
+{
+    constexpr bool t64 = sizeof(R) == 8;
+    cc_ctx *c0 = b->ctx[C.members[0]];
+    const bool pipe = C.sched.pipeline != 0;
+    hipStream_t s0 = C.s0, sb = pipe ? C.s1 : C.s0;
+    const int S = C.sched.steps_per_launch;
+    const int nchunks = (C.maxsteps + S - 1) / S;
+    while ((int)C.chunk_ev.size() < nchunks) {
+        hipEvent_t e;
+        BCHK(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        C.chunk_ev.push_back(e);
+    }
+    if (pipe) {
+        BCHK(b, hipEventRecord(C.ev_fork, s0));
+        BCHK(b, hipStreamWaitEvent(sb, C.ev_fork, 0));
+    }
+    for (int i : C.members) {
+        cc_ctx *c = b->ctx[i];
+        CCCHK(bstat(b, c, tangent_prologue<N>(c, c->x, s0)));
+        CCCHK(bstat(b, c, launch_prologue<N>(c, c->base, counts_of(c->base_plan), c->x, c->p.EpsNewton, sb)));
+    }
+    ccd::ChainParams<double> ab = chain_params<N>(c0, c0->base);
+    ab.Y = C.Y;
+    ab.start = C.start;
+    ab.nsteps = C.nsteps;
+    ab.dir = C.dir;
+    ab.delta = C.delta;
+    ab.grid = C.grid;
+    ab.goff = C.goff;
+    ab.dim = C.dim;
+    ab.status = C.status;
+    ab.fail_idx = C.fail;
+    ab.its = C.its;
+    ab.rec = C.rec;
+    ab.rec_base = C.rec_base;
+    ccd::TangentParams<R> at = tangent_params<N, R>(c0);
+    at.dY = static_cast<cx<R> *>(C.dY);
+    at.rec = static_cast<const R *>(C.rec);
+    at.rec_base = C.t_rec_base;
+    at.start = C.t_start;
+    at.nsteps = C.t_nsteps;
+    at.dir = C.t_dir;
+    at.base_status = C.status;
+    at.bs_idx = C.t_bs;
+    at.dim = C.t_dim;
+    at.goff = C.t_goff;
+    at.delta = C.delta;
+    at.grid = C.grid;
+    at.status = C.t_status;
+    at.fail_idx = C.t_fail;
+    at.its = C.t_its;
+    for (int k = 0; k < nchunks; ++k) {
+        ab.step_begin = at.step_begin = k * S;
+        ab.step_end = at.step_end = std::min(C.maxsteps, (k + 1) * S);
+        BCHK(b, Ops<N>::chain(sb, C.sched.threads_base, C.nbase, ab, t64));
+        if (pipe) {
+            BCHK(b, hipEventRecord(C.chunk_ev[k], sb));
+            BCHK(b, hipStreamWaitEvent(s0, C.chunk_ev[k], 0));
+        }
+        BCHK(b, Ops<N>::tangent(s0, C.sched.threads_tangent, C.ntan, at));
+    }
+    for (int i : C.members) {
+        cc_ctx *c = b->ctx[i];
+        const Counts kb = counts_of(c->base_plan);
+        BCHK(b, Ops<N>::mismatch(sb, false, kb.ncols, c->base.Y, c->base.lidx, c->base.ridx, c->base.delta,
+                                 c->p.PrecisionIRK, c->tw64, c->base.F, c->base.mstatus));
+        hipLaunchKernelGGL(ccd::k_fd, dim3(1, c->n), dim3(128), 0, sb, c->base.F, c->base.cols, kb.ncols, 0, c->n,
+                           c->p.EpsNewton, c->J);
+        BCHK(b, hipGetLastError());
+    }
+    if (pipe) {
+        BCHK(b, hipEventRecord(C.ev_join, sb));
+        BCHK(b, hipStreamWaitEvent(s0, C.ev_join, 0));
+    }
+    for (int i : C.members) {
+        cc_ctx *c = b->ctx[i];
+        const int n = c->n, nt = (int)c->t_cols_h.size();
+        BCHK(b, Ops<N>::tangent_column(s0, t64, nt, c->dY, c->t_cols, c->t_lidx, c->t_ridx, c->tw64, c->J));
+        CCCHK(bstat(b, c, readback_columns(c, c->base, counts_of(c->base_plan), true, s0)));
+        BCHK(b, hipMemcpyAsync(c->hs.tstatus, c->t_status, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, s0));
+        BCHK(b, hipMemcpyAsync(c->hs.tits, c->t_its, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, s0));
+        BCHK(b, hipMemcpyAsync(c->F0, c->base.F, sizeof(double) * n, hipMemcpyDeviceToDevice, s0));
+        hipLaunchKernelGGL(ccd::k_negate_copy, dim3((n + 255) / 256), dim3(256), 0, s0, c->F0, c->d, n);
+        BCHK(b, hipGetLastError());
+        CCCHK(bstat(b, c, enqueue_solve(c, n, c->J, c->d, c->dx, s0)));
+    }
+    return CC_OK;
+}
+
+cc_status batch_enqueue_class(cc_batch *b, BatchClass &C)
+{
+    cc_status st = CC_OK;
+    const bool t64 = b->ctx[C.members[0]]->p.precision == CC_FP64_TANGENT;
+    dispatch_n(C.N, [&](auto Nc) {
+        constexpr int NN = decltype(Nc)::value;
+        st = t64 ? batch_enqueue_class_n<NN, double>(b, C) : batch_enqueue_class_n<NN, float>(b, C);
+    });
+    return st;
+}
+
+// Every class forked from s_main and joined back: the whole batch step.
+cc_status batch_enqueue(cc_batch *b)
+{
+    BCHK(b, hipEventRecord(b->ev_start, b->s_main));
+    for (size_t k = 0; k < b->classes.size(); ++k) {
+        BatchClass &C = b->classes[k];
+        BCHK(b, hipStreamWaitEvent(C.s0, b->ev_start, 0));
+        CCCHK(batch_enqueue_class(b, C));
+        BCHK(b, hipEventRecord(b->ev_done[k], C.s0));
+        BCHK(b, hipStreamWaitEvent(b->s_main, b->ev_done[k], 0));
+    }
+    return CC_OK;
+}
+
+void batch_drop_graph(cc_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+}
+
+// Runs one batch step on the members' resident x (eager or graph) and
+// evaluates every member's status.
+cc_status batch_run(cc_batch *b, int32_t *status)
+{
+    if (b->use_graph) {
+        if (!b->gexec) {
+            BCHK(b, hipStreamBeginCapture(b->s_main, hipStreamCaptureModeThreadLocal));
+            const cc_status st = batch_enqueue(b);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(b->s_main, &g);
+            if (st != CC_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            BCHK(b, e);
+            const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            BCHK(b, ei);
+        }
+        BCHK(b, hipGraphLaunch(b->gexec, b->s_main));
+    } else {
+        CCCHK(batch_enqueue(b));
+    }
+    BCHK(b, hipStreamSynchronize(b->s_main));
+    b->counters[0] = b->counters[1] = b->counters[2] = 0;
+    cc_status worst = CC_OK;
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        cc_ctx *c = b->ctx[i];
+        c->counters[0] = c->counters[1] = c->counters[2] = 0;
+        cc_status st = eval_tangent(c, nullptr);
+        if (st == CC_OK) st = eval_info(c, nullptr);
+        for (int k = 0; k < 3; ++k) b->counters[k] += c->counters[k];
+        if (status) status[i] = st;
+        if (st != CC_OK && worst == CC_OK) b->err = "problem " + std::to_string(i) + ": " + c->err;
+        if (st > worst) worst = st;
+    }
+    return worst;
+}
+
+cc_status batch_timed(cc_batch *b, double *ms)
+{
+    cc_ctx *c0 = b->ctx[0];
+    BCHK(b, hipEventRecord(c0->tk0, b->s_main));
+    const cc_status st = batch_run(b, nullptr);
+    BCHK(b, hipEventRecord(c0->tk1, b->s_main));
+    BCHK(b, hipEventSynchronize(c0->tk1));
+    float f = 0.0f;
+    BCHK(b, hipEventElapsedTime(&f, c0->tk0, c0->tk1));
+    *ms = f;
+    return st;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -894,7 +1144,9 @@ cc_status tune_stages(cc_ctx *c, std::vector<double> &times)
 // ===========================================================================
 extern "C" {
 
-cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
+// own_tangent = false: no base record and tangent states of its own (a batch
+// member points them into its class's arrays).
+static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
 {
     if (!p || !out) return CC_BADARG;
     *out = nullptr;
@@ -908,7 +1160,7 @@ cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
     if (c->p.PrecisionIRK32 <= 0.0) c->p.PrecisionIRK32 = 2e-6;
     if (c->p.EpsNewton32 <= 0.0) c->p.EpsNewton32 = 1e-3;
     if (c->p.refine_iters <= 0) c->p.refine_iters = 3;
-    if (p->precision == CC_FP64_TANGENT) c->tangent_rtol = 1e-13;
+    if (p->precision == CC_FP64_TANGENT) c->tangent_rtol = N >= 2048 ? 1e-12 : 1e-13;
     c->N = N;
     c->M = N / 2;
     c->n = 3 * N / 4;
@@ -1001,7 +1253,7 @@ cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
             ALLOC(*q, mc);
         ALLOC(B->its, mch);
     }
-    {
+    if (own_tangent) {
         // record and tangent states in the tangent's precision (Rec<N>::STEP values per step)
         const size_t elem = p->precision == CC_FP64_TANGENT ? sizeof(double) : sizeof(float);
         char *recp = nullptr, *dyp = nullptr;
@@ -1099,6 +1351,8 @@ cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out)
     *out = c;
     return CC_OK;
 }
+
+cc_status cc_ctx_create(const cc_problem *p, cc_ctx **out) { return create_ctx(p, out, true); }
 
 cc_status cc_ctx_destroy(cc_ctx *c)
 {
@@ -1533,6 +1787,294 @@ cc_status cc_expansions(cc_ctx *c, const double *x, double *YL, double *YR)
     HIPCHK(c, hipMemcpyAsync(YL, Y, sizeof(cx<double>) * M, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipMemcpyAsync(YR, Y + M, sizeof(cx<double>) * M, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));
+    return CC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched independent problems
+// ---------------------------------------------------------------------------
+cc_status cc_batch_destroy(cc_batch *b)
+{
+    if (!b) return CC_BADARG;
+    if (b->s_main) (void)hipStreamSynchronize(b->s_main);
+    for (BatchClass &C : b->classes) {
+        if (C.s0) (void)hipStreamSynchronize(C.s0);
+        if (C.s1) (void)hipStreamSynchronize(C.s1);
+    }
+    batch_drop_graph(b);
+    for (BatchClass &C : b->classes) {
+        for (hipEvent_t e : C.chunk_ev) (void)hipEventDestroy(e);
+        if (C.ev_fork) (void)hipEventDestroy(C.ev_fork);
+        if (C.ev_join) (void)hipEventDestroy(C.ev_join);
+        if (C.s0) (void)hipStreamDestroy(C.s0);
+        if (C.s1) (void)hipStreamDestroy(C.s1);
+    }
+    for (hipEvent_t e : b->ev_done) (void)hipEventDestroy(e);
+    if (b->ev_start) (void)hipEventDestroy(b->ev_start);
+    if (b->s_main) (void)hipStreamDestroy(b->s_main);
+    for (cc_ctx *c : b->ctx) cc_ctx_destroy(c);
+    for (void *q : b->allocs) (void)hipFree(q);
+    delete b;
+    return CC_OK;
+}
+
+const char *cc_batch_last_error(const cc_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+cc_status cc_batch_create(const cc_problem *problems, int32_t count, cc_batch **out)
+{
+    if (!problems || count < 1 || !out) return CC_BADARG;
+    *out = nullptr;
+    const cc_problem &p0 = problems[0];
+    for (int i = 0; i < count; ++i) {
+        const cc_problem &p = problems[i];
+        if ((p.precision != CC_MIXED && p.precision != CC_FP64_TANGENT) || p.precision != p0.precision ||
+            p.MaxIterIRK != p0.MaxIterIRK || p.PrecisionIRK != p0.PrecisionIRK || p.EpsNewton != p0.EpsNewton ||
+            p.refine_iters != p0.refine_iters)
+            return CC_BADARG;
+    }
+    cc_batch *b = new cc_batch();
+    auto bail = [&](cc_status st) {
+        cc_batch_destroy(b);
+        return st;
+    };
+    for (int i = 0; i < count; ++i) {
+        cc_ctx *c = nullptr;
+        const cc_status st = create_ctx(&problems[i], &c, false);
+        if (st != CC_OK) return bail(st);
+        b->ctx.push_back(c);
+    }
+    if (bfail(b, "stream", hipStreamCreateWithFlags(&b->s_main, hipStreamNonBlocking))) return bail(CC_DEVICE);
+    if (bfail(b, "event", hipEventCreateWithFlags(&b->ev_start, hipEventDisableTiming))) return bail(CC_DEVICE);
+    for (int Nv : {128, 256, 512, 1024, 2048}) {
+        BatchClass C;
+        C.N = Nv;
+        for (int i = 0; i < count; ++i)
+            if (problems[i].Ntau == Nv) C.members.push_back(i);
+        if (!C.members.empty()) b->classes.push_back(C);
+    }
+    const bool t64 = p0.precision == CC_FP64_TANGENT;
+    const size_t elem = t64 ? sizeof(double) : sizeof(float);
+    for (BatchClass &C : b->classes) {
+        const int N = C.N, M = N / 2;
+        const size_t step = (size_t)(2 * (5 * (N / 2) + 2 * N + 4));       // Rec<N>::STEP
+        hipEvent_t e;
+        if (bfail(b, "stream", hipStreamCreateWithFlags(&C.s0, hipStreamNonBlocking))) return bail(CC_DEVICE);
+        if (bfail(b, "stream", hipStreamCreateWithFlags(&C.s1, hipStreamNonBlocking))) return bail(CC_DEVICE);
+        if (bfail(b, "event", hipEventCreateWithFlags(&C.ev_fork, hipEventDisableTiming))) return bail(CC_DEVICE);
+        if (bfail(b, "event", hipEventCreateWithFlags(&C.ev_join, hipEventDisableTiming))) return bail(CC_DEVICE);
+        if (bfail(b, "event", hipEventCreateWithFlags(&e, hipEventDisableTiming))) return bail(CC_DEVICE);
+        b->ev_done.push_back(e);
+        size_t ngrid = 0, nrec = 0;
+        for (int i : C.members) {
+            cc_ctx *c = b->ctx[i];
+            C.ntan += (int)c->t_cols_h.size();
+            ngrid += (size_t)c->X + 1;
+            nrec += (size_t)c->X;
+            C.maxsteps = std::max(C.maxsteps, max_chain_steps(c));
+        }
+        C.nbase = 4 * (int)C.members.size();
+        hipError_t e2 = hipSuccess;
+        char *dyp = nullptr, *recp = nullptr;
+#define BALLOC(ptr, cnt) \
+    if (e2 == hipSuccess) e2 = balloc(b, &(ptr), (cnt));
+        BALLOC(C.Y, (size_t)C.nbase * M);
+        BALLOC(C.delta, C.nbase);
+        BALLOC(C.dim, C.nbase);
+        BALLOC(C.grid, ngrid);
+        for (int32_t **q : {&C.start, &C.nsteps, &C.dir, &C.status, &C.fail, &C.goff}) BALLOC(*q, C.nbase);
+        BALLOC(C.its, C.nbase);
+        BALLOC(C.rec_base, C.nbase);
+        BALLOC(dyp, (size_t)C.ntan * M * 2 * elem);
+        BALLOC(recp, nrec * step * elem);
+        BALLOC(C.t_dim, C.ntan);
+        for (int32_t **q : {&C.t_start, &C.t_nsteps, &C.t_dir, &C.t_status, &C.t_fail, &C.t_goff, &C.t_bs})
+            BALLOC(*q, C.ntan);
+        BALLOC(C.t_its, C.ntan);
+        BALLOC(C.t_rec_base, C.ntan);
+#undef BALLOC
+        if (bfail(b, "hipMalloc", e2)) return bail(CC_DEVICE);
+        C.dY = dyp;
+        C.rec = recp;
+        // point every member into the class arrays; per-chain tables for the shared launches
+        std::vector<double> dim_b, dim_t;
+        std::vector<int32_t> goff_b, goff_t, bs_t, tstart, tnsteps, tdir;
+        std::vector<int64_t> rb_b, rb_t;
+        size_t go = 0, ro = 0;
+        int to = 0;
+        for (size_t m = 0; m < C.members.size(); ++m) {
+            cc_ctx *c = b->ctx[C.members[m]];
+            const int bo = 4 * (int)m;
+            if (bfail(b, "grid", hipMemcpy(C.grid + go, c->grid, sizeof(double) * (c->X + 1), hipMemcpyDeviceToDevice)))
+                return bail(CC_DEVICE);
+            c->grid = C.grid + go;
+            c->base.Y = C.Y + (size_t)bo * M;
+            c->base.delta = C.delta + bo;
+            c->base.start = C.start + bo;
+            c->base.nsteps = C.nsteps + bo;
+            c->base.dir = C.dir + bo;
+            c->base.status = C.status + bo;
+            c->base.fail = C.fail + bo;
+            c->base.its = C.its + bo;
+            if (bfail(b, "plan", upload_plan(c->base_plan, c->base))) return bail(CC_DEVICE);
+            c->rec = recp + ro * step * elem;
+            c->dY = dyp + (size_t)to * M * 2 * elem;
+            c->t_status = C.t_status + to;
+            c->t_fail = C.t_fail + to;
+            c->t_its = C.t_its + to;
+            c->t_start = C.t_start + to;
+            c->t_nsteps = C.t_nsteps + to;
+            c->t_dir = C.t_dir + to;
+            for (int k = 0; k < 4; ++k) {
+                dim_b.push_back(c->p.Dim);
+                goff_b.push_back((int32_t)go);
+            }
+            rb_b.insert(rb_b.end(), {(int64_t)ro, (int64_t)(ro + c->iM), -1, -1});
+            for (size_t j = 0; j < c->t_cols_h.size(); ++j) {
+                const bool left = c->t_cols_h[j] >= N / 4;
+                dim_t.push_back(c->p.Dim);
+                goff_t.push_back((int32_t)go);
+                bs_t.push_back(bo);
+                tstart.push_back(left ? 0 : c->iR);
+                tnsteps.push_back(c->t_nsteps_h[j]);
+                tdir.push_back(left ? 1 : -1);
+                rb_t.push_back((int64_t)(ro + (left ? 0 : c->iM)));
+            }
+            go += (size_t)c->X + 1;
+            ro += (size_t)c->X;
+            to += (int)c->t_cols_h.size();
+        }
+        hipError_t eu = up(dim_b, C.dim);
+        if (eu == hipSuccess) eu = up(goff_b, C.goff);
+        if (eu == hipSuccess) eu = up(rb_b, C.rec_base);
+        if (eu == hipSuccess) eu = up(dim_t, C.t_dim);
+        if (eu == hipSuccess) eu = up(goff_t, C.t_goff);
+        if (eu == hipSuccess) eu = up(bs_t, C.t_bs);
+        if (eu == hipSuccess) eu = up(tstart, C.t_start);
+        if (eu == hipSuccess) eu = up(tnsteps, C.t_nsteps);
+        if (eu == hipSuccess) eu = up(tdir, C.t_dir);
+        if (eu == hipSuccess) eu = up(rb_t, C.t_rec_base);
+        if (bfail(b, "plan", eu)) return bail(CC_DEVICE);
+        C.sched = b->ctx[C.members[0]]->sched;
+    }
+    *out = b;
+    return CC_OK;
+}
+
+cc_status cc_batch_newton_step(cc_batch *b, const double *const *x, double *const *F0, double *const *dx,
+                               int32_t *status)
+{
+    if (!b || !x) return CC_BADARG;
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        if (!x[i]) return CC_BADARG;
+        BCHK(b, hipMemcpy(b->ctx[i]->x, x[i], sizeof(double) * b->ctx[i]->n, hipMemcpyHostToDevice));
+    }
+    const cc_status st = batch_run(b, status);
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        cc_ctx *c = b->ctx[i];
+        if (F0 && F0[i]) BCHK(b, hipMemcpy(F0[i], c->F0, sizeof(double) * c->n, hipMemcpyDeviceToHost));
+        if (dx && dx[i]) BCHK(b, hipMemcpy(dx[i], c->dx, sizeof(double) * c->n, hipMemcpyDeviceToHost));
+    }
+    return st;
+}
+
+cc_status cc_batch_info(cc_batch *b, int32_t *nproblems, int32_t *nclasses)
+{
+    if (!b) return CC_BADARG;
+    if (nproblems) *nproblems = (int32_t)b->ctx.size();
+    if (nclasses) *nclasses = (int32_t)b->classes.size();
+    return CC_OK;
+}
+
+cc_status cc_batch_get_schedule(cc_batch *b, int32_t cls, int32_t *Ntau, cc_schedule *out)
+{
+    if (!b || cls < 0 || cls >= (int32_t)b->classes.size() || !out) return CC_BADARG;
+    if (Ntau) *Ntau = b->classes[cls].N;
+    *out = b->classes[cls].sched;
+    out->use_graph = b->use_graph;
+    return CC_OK;
+}
+
+cc_status cc_batch_set_schedule(cc_batch *b, int32_t cls, const cc_schedule *s)
+{
+    if (!b || cls < 0 || cls >= (int32_t)b->classes.size() || !s) return CC_BADARG;
+    BatchClass &C = b->classes[cls];
+    cc_ctx *c0 = b->ctx[C.members[0]];
+    const int T0 = ccd::default_threads(C.N);
+    const bool t64 = c0->p.precision == CC_FP64_TANGENT;
+    if (!valid_threads(C.N, s->threads_base) || !valid_threads(C.N, s->threads_tangent) ||
+        (t64 && (s->threads_base != T0 || s->threads_tangent != T0)) || s->steps_per_launch < 16 ||
+        s->steps_per_launch > 8192 || (C.maxsteps + s->steps_per_launch - 1) / s->steps_per_launch > kMaxChunks) {
+        b->err = "invalid schedule for this class";
+        return CC_BADARG;
+    }
+    C.sched = *s;
+    C.sched.pipeline = s->pipeline != 0;
+    b->use_graph = s->use_graph != 0;
+    batch_drop_graph(b);
+    return CC_OK;
+}
+
+cc_status cc_batch_counters(cc_batch *b, int64_t out[3])
+{
+    if (!b || !out) return CC_BADARG;
+    for (int i = 0; i < 3; ++i) out[i] = b->counters[i];
+    return CC_OK;
+}
+
+// Per class: thread count of the tangent chains x pipeline x steps_per_launch,
+// timing the class's step alone; then the whole batch eager against graph.
+cc_status cc_batch_autotune(cc_batch *b, const double *const *x, double *times, int32_t ntimes, int32_t *ntried)
+{
+    if (!b || !x) return CC_BADARG;
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        if (!x[i]) return CC_BADARG;
+        BCHK(b, hipMemcpy(b->ctx[i]->x, x[i], sizeof(double) * b->ctx[i]->n, hipMemcpyHostToDevice));
+    }
+    std::vector<double> t;
+    b->use_graph = false;
+    batch_drop_graph(b);
+    cc_ctx *c0 = b->ctx[0];
+    for (BatchClass &C : b->classes) {
+        const bool t64 = b->ctx[C.members[0]]->p.precision == CC_FP64_TANGENT;
+        std::vector<int> Ts = {ccd::default_threads(C.N)};
+        if (!t64 && ccd::alt_threads(C.N)) Ts.push_back(ccd::alt_threads(C.N));
+        cc_schedule best = C.sched;
+        double best_ms = 1e300;
+        for (int T : Ts)
+            for (int pipe : {1, 0})
+                for (int S : {128, 512}) {
+                    if ((C.maxsteps + S - 1) / S > kMaxChunks) continue;
+                    C.sched.threads_tangent = T;
+                    C.sched.pipeline = pipe;
+                    C.sched.steps_per_launch = S;
+                    BCHK(b, hipEventRecord(c0->tk0, C.s0));
+                    CCCHK(batch_enqueue_class(b, C));
+                    BCHK(b, hipEventRecord(c0->tk1, C.s0));
+                    BCHK(b, hipEventSynchronize(c0->tk1));
+                    float f = 0.0f;
+                    BCHK(b, hipEventElapsedTime(&f, c0->tk0, c0->tk1));
+                    t.push_back(f);
+                    if (f < best_ms) {
+                        best_ms = f;
+                        best = C.sched;
+                    }
+                }
+        C.sched = best;
+    }
+    double eager = 0.0, graph = 0.0;
+    CCCHK(batch_timed(b, &eager));
+    b->use_graph = true;
+    double dummy = 0.0;
+    cc_status st = batch_timed(b, &dummy);          // capture + first replay
+    if (st == CC_OK) st = batch_timed(b, &graph);
+    if (st != CC_OK) return st;
+    t.push_back(eager);
+    t.push_back(graph);
+    b->use_graph = graph < eager;
+    if (!b->use_graph) batch_drop_graph(b);
+    if (times)
+        for (int i = 0; i < std::min<int>(ntimes, (int)t.size()); ++i) times[i] = t[i];
+    if (ntried) *ntried = (int32_t)t.size();
     return CC_OK;
 }
 

@@ -54,6 +54,8 @@ struct ChainParams {
     double Dim;
     double precision;
     double a00, a01, a10, a11, b0, b1, c0, c1;   // Butcher tableau
+    const double *dim;             // per-chain D (batched problems), null: Dim for every chain
+    const int32_t *goff;           // per-chain offset of its grid in `grid`, null: 0
     void *rec;                     // base-trajectory record (tangent-linear columns), may be null;
                                    // element type RT of k_chain
     const int64_t *rec_base;       // per chain: first record slot, < 0 = do not record
@@ -316,14 +318,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
     cx<R> yk[2][H];
 #pragma unroll
     for (int h = 0; h < H; ++h) ys[0][t + T * h] = Yg[t + T * h];
-    const RhsConst<R> K = make_const<N, R>(a.Dim, a.delta[ch]);
+    const RhsConst<R> K = make_const<N, R>(a.dim ? a.dim[ch] : a.Dim, a.delta[ch]);
+    const double *grid = a.grid + (a.goff ? a.goff[ch] : 0);
 
     const int dir = a.dir[ch];
     int idx = a.start[ch] + dir * s_begin;
     int64_t its_sum = 0;
     int32_t st = 0;
     for (int step = s_begin; step < s_end; ++step, idx += dir) {
-        const double xin = a.grid[idx], xout = a.grid[idx + dir];
+        const double xin = grid[idx], xout = grid[idx + dir];
         const double dxd = xout - xin;
         const R xs[2] = {(R)(xin + dxd * a.c0), (R)(xin + dxd * a.c1)};
         const R A00 = (R)(dxd * a.a00), A01 = (R)(dxd * a.a01), A10 = (R)(dxd * a.a10), A11 = (R)(dxd * a.a11);

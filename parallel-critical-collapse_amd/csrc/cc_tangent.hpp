@@ -36,7 +36,11 @@ struct TangentParams {
     const int64_t *rec_base;  // per chain: record slot of its step 0 (the base chain of its side)
     const int32_t *start, *nsteps, *dir;
     const int32_t *base_status;  // status of the base chains (left, right), may be null
-    const double *delta;      // delta[0]: Delta of the base; every tangent column shares it
+    const int32_t *bs_idx;       // per chain: index of its problem's base-left chain in base_status
+                                 // and delta (batched problems), null: 0
+    const double *dim;           // per-chain D (batched problems), null: Dim
+    const int32_t *goff;         // per-chain grid offset (batched problems), null: 0
+    const double *delta;      // delta[bs_idx]: Delta of the base; every tangent column of a problem shares it
                               // (the Delta column itself is a finite difference in FP64)
     const double *grid;
     int32_t *status, *fail_idx;
@@ -228,7 +232,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
     if (a.status[ch] != 0) return;
     const int dir = a.dir[ch];
     // a failed base chain leaves no record to linearise around
-    if (a.base_status && a.base_status[dir > 0 ? 0 : 1] != 0) return;
+    const int bsi = a.bs_idx ? a.bs_idx[ch] : 0;
+    if (a.base_status && a.base_status[bsi + (dir > 0 ? 0 : 1)] != 0) return;
     const int nst = a.nsteps[ch];
     const int s_begin = a.step_begin;
     const int s_end = min(a.step_end, nst);
@@ -236,13 +241,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
     cx<R> *Yg = a.dY + (size_t)ch * M;
 #pragma unroll
     for (int h = 0; h < H; ++h) ys[0][t + T * h] = Yg[t + T * h];
-    const RhsConst<R> K = make_const<N, R>(a.Dim, a.delta[0]);
+    const RhsConst<R> K = make_const<N, R>(a.dim ? a.dim[ch] : a.Dim, a.delta[bsi]);
+    const double *grid = a.grid + (a.goff ? a.goff[ch] : 0);
     int idx = a.start[ch] + dir * s_begin;
     int64_t its_sum = 0;
     int32_t st = 0;
     cx<R> dk[2][H];
     for (int step = s_begin; step < s_end; ++step, idx += dir) {
-        const double xin = a.grid[idx], xout = a.grid[idx + dir];
+        const double xin = grid[idx], xout = grid[idx + dir];
         const double dxd = xout - xin;
         const R xs[2] = {(R)(xin + dxd * a.c0), (R)(xin + dxd * a.c1)};
         const R A00 = (R)(dxd * a.a00), A01 = (R)(dxd * a.a01), A10 = (R)(dxd * a.a10),
@@ -284,8 +290,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
             const R d2 = block_sum<T, R>(loc, red, t);
             const R m2 = block_sum<T, R>(mag, red, t);
             __syncthreads();
-            // linear fixed point: stop when the update is below rtol of the iterate
-            if (d2 <= (R)(a.rtol * a.rtol) * m2) {
+            // linear fixed point: stop when the update is below rtol of the
+            // iterate, relaxed late as the base iteration's tolerance
+            // (ODEStepper.cpp:265-269)
+            double rtol = a.rtol;
+            if (its > a.maxIts / 2) rtol = a.rtol * 10.0 * (2.0 * (double)its / (double)a.maxIts - 0.5);
+            if (d2 <= (R)(rtol * rtol) * m2) {
                 conv = true;
                 break;
             }
