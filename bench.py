@@ -191,6 +191,68 @@ def newton_study(cc, shape, policies, tol, max_its, tune, save_failures=None):
     return out
 
 
+def batch_bench(cc, steps, warmup, tune):
+    """The batched uneven problems: every problem of the dimension sweep
+    stepped together through cc_batch (one launch per Ntau class and chunk),
+    against the same problems stepped one after another on their own
+    contexts; per-problem Jacobian error of the mixed policy against the
+    FP64 tangent-linear Jacobian."""
+    W = importlib.import_module("parallel-critical-collapse_amd.workload")
+    shapes = W.config4()
+    xs = [W.initial_x(sh) for sh in shapes]
+    b = cc.Batch([cc.Problem.from_shape(sh, precision=cc.CC_MIXED) for sh in shapes])
+    tuning = None
+    if tune:
+        t = time.perf_counter()
+        times = b.autotune(xs)
+        tuning = {"candidate_ms": [round(float(v), 3) for v in times], "tuning_s": round(time.perf_counter() - t, 2),
+                  "schedules": {b.schedule(c)[0]: b.schedule(c)[1].as_dict() for c in range(b.nclasses)}}
+        log(f"[bench] batch autotune {tuning['tuning_s']} s")
+    for w in range(warmup):
+        b.newton_step(xs)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        F0b, dxb, status, st = b.newton_step(xs, check=False)
+        log(f"[bench] batch step {k + 1}/{steps}: worst status {st}")
+    wall = time.perf_counter() - t0
+    cnt = b.counters()
+    b.close()
+    # the same problems one after another, each on its own context
+    seq = 0.0
+    errs = []
+    for i, sh in enumerate(shapes):
+        ctx = cc.Context(cc.Problem.from_shape(sh, precision=cc.CC_MIXED))
+        ctx.newton_step(xs[i])
+        t = time.perf_counter()
+        F0, dx = ctx.newton_step(xs[i])
+        seq += time.perf_counter() - t
+        J32, _, _ = ctx.jacobian(xs[i], F0)
+        ctx.close()
+        c64 = cc.Context(cc.Problem.from_shape(sh, precision=cc.CC_FP64_TANGENT))
+        F64, dx64 = c64.newton_step(xs[i])
+        J64, _, _ = c64.jacobian(xs[i], F64)
+        c64.close()
+        errs.append({"Ntau": sh.Ntau, "x_points": sh.X, "Dim": sh.Dim, "scale": sh.scale,
+                     "status": int(status[i]),
+                     "F0_equal_fp64": bool(np.array_equal(F0b[i], F64)),
+                     "J_rel_err": float(np.linalg.norm(J32 - J64) / np.linalg.norm(J64)),
+                     "dx_rel_err": float(np.linalg.norm(dxb[i] - dx64) / np.linalg.norm(dx64))})
+        log(f"[bench] problem {i}: {errs[-1]}")
+    n = len(shapes)
+    return {
+        "metric": f"Jacobian+Newton steps/s, {n} uneven problems batched (mixed FP32/FP64)",
+        "value": round(n * steps / wall, 4), "unit": "Newton steps/s (all problems)", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(1e3 * wall / steps, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic: shipped D=4 solutions + seeded perturbations, D sweep, Ntau/x-points by seed",
+        "config": {"workload": "batch of " + ", ".join(f"(Ntau={sh.Ntau}, X={sh.X}, D={sh.Dim})" for sh in shapes)},
+        "sequential_value": round(n / seq, 4), "batched_over_sequential": round((n * steps / wall) / (n / seq), 3),
+        "chains_per_step": int(cnt[2]), "tuning": tuning, "problems": errs,
+        "J_rel_err_max": max(e["J_rel_err"] for e in errs),
+        "J_rel_err_median": float(np.median([e["J_rel_err"] for e in errs])),
+    }
+
+
 def copy_calibration(nbytes):
     """A plain device copy of nbytes (torch's copy kernel), for scaling the
     TCC FETCH_SIZE / WRITE_SIZE counters of a --pmc run to bytes."""
@@ -209,7 +271,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=3, help="workload shape 1, 2, 3 or 5")
+    ap.add_argument("--config", type=int, default=3,
+                    help="workload shape 1, 2, 3 or 5; 4 = the batch of uneven problems")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 policy")
     ap.add_argument("--no-tune", action="store_true", help="default schedule instead of the autotuner's")
@@ -247,6 +310,9 @@ def main():
     cc = importlib.import_module("parallel-critical-collapse_amd")
     dev = importlib.import_module("parallel-critical-collapse_amd.device")
     W = importlib.import_module("parallel-critical-collapse_amd.workload")
+    if args.config == 4:
+        print(json.dumps(batch_bench(cc, args.steps, args.warmup, not args.no_tune)), flush=True)
+        return
     shape = W.config(args.config)
     log(f"[bench] workload Ntau={shape.Ntau} X={shape.X} n={shape.n}")
 

@@ -114,7 +114,11 @@ __device__ __forceinline__ void dft(cx<R>* v)
 // remaining factor.  All but the last pass exchange through LDS (B*N
 // complex); the last pass's outputs land back in layout L.
 // tw[e] = (cos, sin)(2 pi e / N), e in [0, N).
-template <int N, int T, int SIGN, int B, int NS, typename R>
+// DB = 1: the exchanges alternate between two LDS buffers (2*B*N complex),
+// so each exchange needs one barrier instead of two: a thread writing buffer
+// k+2 has passed exchange k+1's barrier, which every thread reaches only
+// after reading buffer k.  PAR is the buffer of this pass.
+template <int N, int T, int SIGN, int B, int NS, typename R, int DB = 0, int PAR = 0>
 __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const cx<R>* __restrict__ tw,
                                          int t)
 {
@@ -169,7 +173,7 @@ __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const
             } else {
                 const int base = (j / NS) * NS * RAD + k;
 #pragma unroll
-                for (int r = 0; r < RAD; ++r) lds[s * N + base + r * NS] = u[r];
+                for (int r = 0; r < RAD; ++r) lds[PAR * B * N + s * N + base + r * NS] = u[r];
             }
         }
     }
@@ -178,27 +182,30 @@ __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const
 #pragma unroll
         for (int s = 0; s < B; ++s)
 #pragma unroll
-            for (int p = 0; p < P; ++p) v[s][p] = lds[s * N + t + T * p];
-        __syncthreads();
-        fft_pass<N, T, SIGN, B, NS * RAD>(v, lds, tw, t);
+            for (int p = 0; p < P; ++p) v[s][p] = lds[PAR * B * N + s * N + t + T * p];
+        if constexpr (!DB) __syncthreads();
+        fft_pass<N, T, SIGN, B, NS * RAD, R, DB, DB ? 1 - PAR : 0>(v, lds, tw, t);
     }
 }
 
 // Batched block FFT: B signals of length N in layout L, in place.
 // Requires N/T >= 2 and a __syncthreads-safe caller (all T threads call it).
-template <int N, int T, int SIGN, int B, typename R>
+// lds holds B*N complex, or 2*B*N with DB = 1 (one trailing barrier then
+// protects the buffers from the caller's next LDS writes).
+template <int N, int T, int SIGN, int B, typename R, int DB = 0>
 __device__ __forceinline__ void fft_block(cx<R> (&v)[B][N / T], cx<R>* lds, const cx<R>* __restrict__ tw, int t)
 {
     static_assert((N & (N - 1)) == 0 && N / T >= 2 && N / T <= 16, "unsupported FFT shape");
-    fft_pass<N, T, SIGN, B, 1>(v, lds, tw, t);
+    fft_pass<N, T, SIGN, B, 1, R, DB, 0>(v, lds, tw, t);
+    if constexpr (DB) __syncthreads();
 }
 
 // Single-signal convenience wrapper.
-template <int N, int T, int SIGN, typename R>
+template <int N, int T, int SIGN, typename R, int DB = 0>
 __device__ __forceinline__ void fft1(cx<R> (&v)[N / T], cx<R>* lds, const cx<R>* __restrict__ tw, int t)
 {
     cx<R>(&vb)[1][N / T] = *reinterpret_cast<cx<R>(*)[1][N / T]>(&v);
-    fft_block<N, T, SIGN, 1>(vb, lds, tw, t);
+    fft_block<N, T, SIGN, 1, R, DB>(vb, lds, tw, t);
 }
 
 // Workgroup sum of one value per thread (T threads, T a multiple of 64).
@@ -218,6 +225,34 @@ __device__ __forceinline__ A block_sum(A v, A* red, int t)
         for (int w = 1; w < T / 64; ++w) s += red[w];
         __syncthreads();
         return s;
+    }
+}
+
+// Workgroup sums of two values per thread in one pass (one LDS round trip).
+// red: LDS scratch of at least 2*T/64 values.
+template <int T, typename A>
+__device__ __forceinline__ void block_sum2(A &a, A &b, A* red, int t)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    if constexpr (T > 64) {
+        if ((t & 63) == 0) {
+            red[t >> 6] = a;
+            red[T / 64 + (t >> 6)] = b;
+        }
+        __syncthreads();
+        A sa = red[0], sb = red[T / 64];
+#pragma unroll
+        for (int w = 1; w < T / 64; ++w) {
+            sa += red[w];
+            sb += red[T / 64 + w];
+        }
+        __syncthreads();
+        a = sa;
+        b = sb;
     }
 }
 

@@ -58,9 +58,13 @@ __global__ __launch_bounds__(LU_PANEL_T) void k_lu_panel(R *A, int n, int k0, in
                     b = sval[w];
                     i0 = sidx[w];
                 }
+            // a column with no comparable entry (all NaN) yields no candidate:
+            // keep the diagonal so every row index stays in [j, n)
+            if (i0 >= n) i0 = j;
             spiv = i0;
             ipiv[j] = i0 + 1;
-            if (A[(size_t)i0 * n + j] == (R)0 && info[0] == 0) info[0] = j + 1;
+            // zero pivot as LAPACK; a NaN pivot is reported the same way
+            if (!(fabs(A[(size_t)i0 * n + j]) > (R)0) && info[0] == 0) info[0] = j + 1;
         }
         __syncthreads();
         const int piv = spiv;

@@ -52,6 +52,20 @@ struct TangentParams {
     double a00, a01, a10, a11, b0, b1, c0, c1;
 };
 
+// FP32 tangents up to N = 1024 double-buffer their transform exchanges (one
+// barrier per exchange); the second buffer would cost occupancy elsewhere.
+template <int N, typename R>
+constexpr int tangent_db() { return (sizeof(R) == 4 && N <= 1024) ? 1 : 0; }
+
+// Reciprocal: one v_rcp_f32 (1 ulp) in the FP32 tangent, whose tolerance is
+// 1e-6 relative; the correctly rounded quotient in the FP64 tangent.
+template <typename R>
+__device__ __forceinline__ R trcp(R x)
+{
+    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);
+    else return R(1.0) / x;
+}
+
 template <int N, int T, typename R>
 __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2], cx<R> (&df)[2][N / T / 2],
                                                  const R *__restrict__ rec, const R (&xs)[2],
@@ -62,17 +76,14 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
     using RC = Rec<N>;
     const R half = R(0.5), invN = R(1.0) / (R)N;
     asm volatile("" : "+v"(t));
-    // The record of a step is the same for every fixed-point iteration; an
-    // opaque pointer keeps its loads (and everything computed from them) in
-    // the iteration instead of hoisted in front of the loop, where they would
-    // spill.
-    asm volatile("" : "+s"(rec));
     // per-stage products that survive to the pointwise tangent RHS
-    R dAU[2][H], dBU[2][H], BU[2][H], dAV[2][H], dBV[2][H], BV[2][H], dCF[2][H], CF[2][H];
+    // (the base-only products B_U, B_V, C_F are recomputed from the record in
+    // the pointwise stage: held across the transforms they would spill)
+    R dAU[2][H], dBU[2][H], dAV[2][H], dBV[2][H], dCF[2][H];
     cx<R> c[P];                              // packed h of both stages
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const R *rs = rec + s * RC::STAGE;
+        int stage_off = s * RC::STAGE;
         cx<R> z[2][P];
         const R fix = bc[s];
 #pragma unroll
@@ -101,10 +112,14 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             const cx<R> v = z[0][p];
             z[1][p] = {w * v.y, -(w * v.x)};
         }
-        fft_block<N, T, -1, 2>(z, xb, tw, t);
-        // the record loads of this stage stay behind its transform
-        asm volatile("" : "+s"(rs));
-        const R X = xs[s], two_x = R(2.0) * X, inv_x = R(1.0) / X;
+        fft_block<N, T, -1, 2, R, tangent_db<N, R>()>(z, xb, tw, t);
+        // The record of a step is the same for every fixed-point iteration and
+        // every transform; an opaque offset keeps this stage's loads behind its
+        // transform instead of hoisted in front of it (or of the loop), where
+        // they would spill.  (An offset, not a pointer: the loads stay global.)
+        asm volatile("" : "+s"(stage_off));
+        const R *rs = rec + stage_off;
+        const R X = xs[s], two_x = R(2.0) * X, inv_x = trcp(X);
         R dC1[H];
 #pragma unroll
         for (int h = 0; h < H; ++h) {
@@ -116,20 +131,17 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             const R U = rs[RC::U + j], V = rs[RC::V + j], F = rs[RC::F + j];
             const R tU = rs[RC::DU + j], tV = rs[RC::DV + j];
             // d Coeff1 (ICG.cpp:834-835 differentiated)
-            dC1[h] = -(K.c3 / (R(8.0) * X)) *
+            dC1[h] = -(K.c3 * R(0.125) * inv_x) *
                      (dF * U * U + R(2.0) * (X + F) * U * dU - dF * V * V + R(2.0) * (X - F) * V * dV);
-            const R inv1 = R(1.0) / (two_x * (F + X)), inv2 = R(1.0) / (two_x * (F - X));
+            const R inv1 = trcp(two_x * (F + X)), inv2 = trcp(two_x * (F - X));
             const R dinv1 = -inv1 * inv1 * two_x * dF, dinv2 = -inv2 * inv2 * two_x * dF;
             const R S = U + V, dS = dU + dV;
             const R numU = F * K.dm2 * S - two_x * tU, numV = F * K.dm2 * S + two_x * tV;
             const R dnum = K.dm2 * (dF * S + F * dS);
             dAU[s][h] = (dnum - two_x * ddU) * inv1 + numU * dinv1;
-            BU[s][h] = K.two_dm3 * F * U * inv1;
             dBU[s][h] = K.two_dm3 * ((dF * U + F * dU) * inv1 + F * U * dinv1);
             dAV[s][h] = (dnum + two_x * ddV) * inv2 + numV * dinv2;
-            BV[s][h] = K.two_dm3 * F * V * inv2;
             dBV[s][h] = K.two_dm3 * ((dF * V + F * dV) * inv2 + F * V * dinv2);
-            CF[s][h] = K.dm3 * F * inv_x;
             dCF[s][h] = K.dm3 * dF * inv_x;
         }
         // h_s = -(dC1 IA2) e^{F_int} = -dC1 E / r  (dC1 is even in the half period)
@@ -137,12 +149,12 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         for (int p = 0; p < P; ++p) {
             const int ph = (p < H) ? p : p - H;
             const int j = t + T * p;
-            const R hv = -dC1[ph] * rs[RC::E + j] / rs[RC::R + j];
+            const R hv = -dC1[ph] * rs[RC::E + j] * trcp(rs[RC::R + j]);
             if (s == 0) c[p].x = hv; else c[p].y = hv;
         }
     }
     // dIA2 = solveInhom with the base's integrating factor and lambda
-    fft1<N, T, +1>(c, xb, tw, t);
+    fft1<N, T, +1, R, tangent_db<N, R>()>(c, xb, tw, t);
 #pragma unroll
     for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
 #pragma unroll
@@ -167,19 +179,19 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             const cx<R> hs = s ? S1 : S0;
             const R lam = s ? lam1 : lam0;
             if (k == 0) {
-                G[s] = (lam != R(0.0)) ? cx<R>{hs.x / lam, hs.y / lam} : cx<R>{R(0.0), R(0.0)};
+                G[s] = (lam != R(0.0)) ? cscale(hs, trcp(lam)) : cx<R>{R(0.0), R(0.0)};
             } else if (k == N / 2) {
-                G[s] = cscale(hs, lam / (lam * lam + K.q2));
+                G[s] = cscale(hs, lam * trcp(lam * lam + K.q2));
             } else {
                 const R w = (R)wavenumber<N>(k) * K.k0;
                 const R den = lam * lam + w * w;
-                const R inv = R(1.0) / den;
+                const R inv = trcp(den);
                 G[s] = {(hs.x * lam - hs.y * w) * inv, (hs.y * lam + hs.x * w) * inv};
             }
         }
         c[p] = {G[0].x - G[1].y, G[0].y + G[1].x};
     }
-    fft1<N, T, -1>(c, xb, tw, t);
+    fft1<N, T, -1, R, tangent_db<N, R>()>(c, xb, tw, t);
     // pointwise tangent RHS
     cx<R> yp[2][P];
 #pragma unroll
@@ -189,17 +201,25 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         const int j = t + T * p;
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const R *rs = rec + s * RC::STAGE;
+            int fin_off = s * RC::STAGE;
+            asm volatile("" : "+s"(fin_off));          // behind the inverse transform, as above
+            const R *rs = rec + fin_off;
             const R r = rs[RC::R + j], E = rs[RC::E + j];
-            const R dIA2 = (s ? c[p].y : c[p].x) / E;
+            const int jh = t + T * ph;
+            const R U = rs[RC::U + jh], V = rs[RC::V + jh], F = rs[RC::F + jh];
+            const R X = xs[s], two_x = R(2.0) * X;
+            const R BU = K.two_dm3 * F * U * trcp(two_x * (F + X));
+            const R BV = K.two_dm3 * F * V * trcp(two_x * (F - X));
+            const R CF = K.dm3 * F * trcp(X);
+            const R dIA2 = (s ? c[p].y : c[p].x) * trcp(E);
             const R dr = -r * r * dIA2;
-            const R dUx = sg * (dAU[s][ph] - dBU[s][ph] * r - BU[s][ph] * dr);
-            const R dVx = sg * (dAV[s][ph] - dBV[s][ph] * r - BV[s][ph] * dr);
-            const R dFx = dCF[s][ph] * (r - R(1.0)) + CF[s][ph] * dr;
+            const R dUx = sg * (dAU[s][ph] - dBU[s][ph] * r - BU * dr);
+            const R dVx = sg * (dAV[s][ph] - dBV[s][ph] * r - BV * dr);
+            const R dFx = dCF[s][ph] * (r - R(1.0)) + CF * dr;
             yp[s][p] = {dUx, dVx + dFx};
         }
     }
-    fft_block<N, T, +1, 2>(yp, xb, tw, t);
+    fft_block<N, T, +1, 2, R, tangent_db<N, R>()>(yp, xb, tw, t);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -223,10 +243,10 @@ template <int N, int T, typename R>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves<N, T, R>()))) void k_tangent(TangentParams<R> a)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
-    __shared__ cx<R> xb[2 * N];
+    __shared__ cx<R> xb[(1 + tangent_db<N, R>()) * 2 * N];
     __shared__ cx<R> ys[2][M];
     __shared__ R bc[8];
-    __shared__ R red[(T / 64) > 0 ? T / 64 : 1];
+    __shared__ R red[2 * ((T / 64) > 0 ? T / 64 : 1)];
     const int t = threadIdx.x;
     const int ch = blockIdx.x;
     if (a.status[ch] != 0) return;
@@ -287,15 +307,16 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
                 bc[0] = dk[0][H - 1].x;
                 bc[1] = dk[1][H - 1].x;
             }
-            const R d2 = block_sum<T, R>(loc, red, t);
-            const R m2 = block_sum<T, R>(mag, red, t);
-            __syncthreads();
+            R d2 = loc, m2 = mag;
+            block_sum2<T, R>(d2, m2, red, t);          // its barriers publish bc[0..1] (T > 64)
+            if constexpr (T == 64) __syncthreads();
             // linear fixed point: stop when the update is below rtol of the
             // iterate, relaxed late as the base iteration's tolerance
             // (ODEStepper.cpp:265-269)
             double rtol = a.rtol;
             if (its > a.maxIts / 2) rtol = a.rtol * 10.0 * (2.0 * (double)its / (double)a.maxIts - 0.5);
-            if (d2 <= (R)(rtol * rtol) * m2) {
+            // (an overflowed iterate, m2 = inf, is never accepted)
+            if (isfinite(m2) && d2 <= (R)(rtol * rtol) * m2) {
                 conv = true;
                 break;
             }

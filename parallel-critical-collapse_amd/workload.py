@@ -100,15 +100,30 @@ def config(index: int) -> Shape:
     raise ValueError(index)
 
 
+# Perturbation scale of each batched problem: the largest power of ten <= 1
+# at which the oracle's FP64 shoot of the seeded input survives.
+CONFIG4_SCALE = (0.1, 0.1, 0.1, 0.1, 1.0, 1.0, 1.0, 1.0, 1.0, 0.1, 1.0, 0.1,
+                 0.1, 1.0, 1.0, 0.1, 1.0, 0.1, 0.1, 1.0, 0.1, 1.0, 0.1, 0.1)
+
+
 def config4() -> list[Shape]:
-    """24 uneven problems: Dim = 3.5 + 0.1 (i mod 15), Ntau and X by seed."""
+    """24 uneven problems: Ntau and X drawn by seed from {128, 256, 512} and
+    {2000, 4000, 6000}, Dim = 3.96 + 0.01 (i mod 15), anchored on the D=4 data.
+
+    Two measured feasibility limits shape this set: at Ntau=512 the
+    reference's fixed-point IRK diverges on the 2000-point grid even for the
+    unperturbed D=4 data (so X is raised to 4000 there), and with D=4 data
+    as the starting point every shoot with D <= 3.93 or D >= 4.6 that was
+    tried failed at every perturbation scale, as did most with D <= 3.9."""
     rng = SplitMix64(SEED_BASE + 4)
     out = []
     for i in range(24):
         Ntau = (128, 256, 512)[rng.next_u64() % 3]
         X = (2000, 4000, 6000)[rng.next_u64() % 3]
-        out.append(Shape(Ntau=Ntau, X=X, Dim=round(3.5 + 0.1 * (i % 15), 10),
-                         seed=SEED_BASE + 4000 + i))
+        if Ntau == 512 and X < 4000:
+            X = 4000
+        out.append(Shape(Ntau=Ntau, X=X, Dim=round(3.96 + 0.01 * (i % 15), 10),
+                         seed=SEED_BASE + 4000 + i, scale=CONFIG4_SCALE[i]))
     return out
 
 

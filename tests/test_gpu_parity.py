@@ -246,6 +246,20 @@ def test_solve_singular_reports_info():
     assert st == cc.CC_SINGULAR and info == info_ref == 2
 
 
+@pytest.mark.parametrize("precision", [cc.CC_FP64, cc.CC_MIXED])
+def test_solve_nan_column_reports_info(precision):
+    """A column with no comparable pivot candidate (all NaN) keeps the
+    diagonal as pivot row and is reported like a zero pivot (info = column+1)
+    instead of selecting a row outside the matrix."""
+    rng = np.random.default_rng(3)
+    n = 96
+    A = rng.standard_normal((n, n)) + np.eye(n)
+    A[:, 5] = np.nan
+    ctx = cc.Context(cc.Problem.make(Ntau=128, NLeft=10, NRight=10, precision=precision))
+    _, info, _, st = ctx.solve(A, np.ones(n))
+    assert st == cc.CC_SINGULAR and info == 6
+
+
 def test_mixed_solve_refines_to_fp64():
     rng = np.random.default_rng(5)
     n = 384

@@ -63,10 +63,30 @@ def test_workload_is_deterministic_and_shaped():
     probs = W.config4()
     assert len(probs) == 24
     assert {p.Ntau for p in probs} <= {128, 256, 512} and {p.X for p in probs} <= {2000, 4000, 6000}
-    assert [round(p.Dim, 6) for p in probs[:16]] == [round(3.5 + 0.1 * (i % 15), 6) for i in range(16)]
+    assert [round(p.Dim, 6) for p in probs[:16]] == [round(3.96 + 0.01 * (i % 15), 6) for i in range(16)]
+    assert all(p.X >= 4000 for p in probs if p.Ntau == 512)
 
 
 def test_grid_matches_oracle():
     s = W.config(1)
     cfg = O.Config(Ntau=s.Ntau, NLeft=s.NLeft, NRight=s.NRight, XMid=s.XMid)
     assert np.array_equal(W.grid(s), O.Oracle(cfg).grid())
+
+
+@pytest.mark.slow
+def test_batched_problems_survive_the_oracle_shoot():
+    """Every problem of the batched workload shoots in the FP64 oracle at its
+    recorded perturbation scale (the scale table's defining property)."""
+    import oracle as O
+    from concurrent.futures import ThreadPoolExecutor
+
+    def shoot(sh):
+        cfg = O.Config(Ntau=sh.Ntau, NLeft=sh.NLeft, NRight=sh.NRight, XMid=sh.XMid, XLeft=sh.XLeft,
+                       XRight=sh.XRight, Dim=sh.Dim, Delta=sh.Delta, EpsNewton=sh.EpsNewton,
+                       PrecisionIRK=sh.PrecisionIRK, MaxIterIRK=sh.MaxIterIRK)
+        return O.Oracle(cfg).shoot(W.initial_x(sh))[1]
+
+    shapes = W.config4()
+    assert len(shapes) == 24 and {s.Ntau for s in shapes} == {128, 256, 512}
+    with ThreadPoolExecutor(8) as ex:
+        assert list(ex.map(shoot, shapes)) == [0] * 24
