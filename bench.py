@@ -232,10 +232,16 @@ def batch_bench(cc, steps, warmup, tune):
         F64, dx64 = c64.newton_step(xs[i])
         J64, _, _ = c64.jacobian(xs[i], F64)
         c64.close()
+        N = sh.Ntau
+        right, left = slice(0, N // 4), slice(N // 4, 3 * N // 4)     # Up columns / psic, fc columns
+
+        def blk(sl):
+            return float(np.linalg.norm(J32[:, sl] - J64[:, sl]) / np.linalg.norm(J64[:, sl]))
         errs.append({"Ntau": sh.Ntau, "x_points": sh.X, "Dim": sh.Dim, "scale": sh.scale,
-                     "status": int(status[i]),
+                     "status": int(status[i]), "mismatch": rms(F64),
                      "F0_equal_fp64": bool(np.array_equal(F0b[i], F64)),
                      "J_rel_err": float(np.linalg.norm(J32 - J64) / np.linalg.norm(J64)),
+                     "J_rel_err_right_side_columns": blk(right), "J_rel_err_left_side_columns": blk(left),
                      "dx_rel_err": float(np.linalg.norm(dxb[i] - dx64) / np.linalg.norm(dx64))})
         log(f"[bench] problem {i}: {errs[-1]}")
     n = len(shapes)
@@ -250,6 +256,7 @@ def batch_bench(cc, steps, warmup, tune):
         "chains_per_step": int(cnt[2]), "tuning": tuning, "problems": errs,
         "J_rel_err_max": max(e["J_rel_err"] for e in errs),
         "J_rel_err_median": float(np.median([e["J_rel_err"] for e in errs])),
+        "J_rel_err_left_side_columns_max": max(e["J_rel_err_left_side_columns"] for e in errs),
     }
 
 

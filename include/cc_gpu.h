@@ -108,6 +108,11 @@ cc_status cc_newton_step(cc_ctx *ctx, const double *x, double *F0, double *dx);
 cc_status cc_newton_step_device_async(cc_ctx *ctx, const double *d_x, double *d_F0, double *d_dx);
 cc_status cc_synchronize(cc_ctx *ctx);
 
+/* Copies the Jacobian assembled by the last cc_jacobian / cc_newton_step*
+ * call (n x n row-major) to host memory: the matrix the reference writes as
+ * jacobian_<Dim>_<i>.csv in Debug runs (NewtonSolver.cpp:337-341). */
+cc_status cc_last_jacobian(cc_ctx *ctx, double *J);
+
 /* Counters of the last Jacobian/Newton launch: [0] IRK steps, [1] fixed-point
  * iterations (sum of itsReached), [2] chains integrated. */
 cc_status cc_counters(cc_ctx *ctx, int64_t out[3]);

@@ -105,6 +105,7 @@ def load():
     L.cc_last_error.restype = C.c_char_p
     L.cc_grid.argtypes = [_vp, _dp]
     L.cc_counters.argtypes = [_vp, _lp]
+    L.cc_last_jacobian.argtypes = [_vp, _dp]
     L.cc_residual.argtypes = [_vp, _dp, _dp]
     L.cc_jacobian.argtypes = [_vp, _dp, _dp, _dp, _ip]
     L.cc_solve.argtypes = [_vp, C.c_int32, _dp, _dp, _ip]
@@ -185,6 +186,11 @@ class Context:
         out = np.zeros(3, np.int64)
         load().cc_counters(self.h, out.ctypes.data_as(_lp))
         return out
+
+    def last_jacobian(self):
+        J = np.zeros((self.n, self.n))
+        self._check(load().cc_last_jacobian(self.h, _d(J)), "cc_last_jacobian")
+        return J
 
     def residual(self, x):
         x = _arr(x)

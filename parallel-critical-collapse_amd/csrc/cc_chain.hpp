@@ -62,17 +62,19 @@ struct ChainParams {
 };
 
 // Base-trajectory record of one IRK step (element type RT of k_chain, the
-// precision of the tangent that reads it): for each stage s, the
-// fields of the last RHS evaluation of the step -- U, V, F, dU/dtau, dV/dtau
+// precision of the tangent that reads it): for each stage s, the fields of
+// the last RHS evaluation of the step -- U, V, F, dU/dtau, dV/dtau and F - x
 // on j < N/2 (the other half follows by symmetry), r = 1/IA2 and
-// E = e^{F_int} on all N points, and lambda = mean(Coeff1).  A
-// tangent-linear chain linearises the RHS around these values.
+// E = e^{F_int} on all N points, and lambda = mean(Coeff1).  F - x is
+// recorded as computed in the base precision: near the horizon F and x agree
+// to three digits, and their difference formed from FP32 copies would keep
+// only four.  A tangent-linear chain linearises the RHS around these values.
 template <int N>
 struct Rec {
     static constexpr int M = N / 2;
-    static constexpr int U = 0, V = M, F = 2 * M, DU = 3 * M, DV = 4 * M, R = 5 * M, E = 5 * M + N,
-                         LAM = 5 * M + 2 * N;
-    static constexpr int STAGE = 5 * M + 2 * N + 4;
+    static constexpr int U = 0, V = M, F = 2 * M, DU = 3 * M, DV = 4 * M, FMX = 5 * M, R = 6 * M, E = 6 * M + N,
+                         LAM = 6 * M + 2 * N;
+    static constexpr int STAGE = 6 * M + 2 * N + 4;
     static constexpr int STEP = 2 * STAGE;
 };
 
@@ -151,6 +153,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
                 rs[RC::F] = (RT)F;
                 rs[RC::DU] = (RT)dU;
                 rs[RC::DV] = (RT)dV;
+                rs[RC::FMX] = (RT)(F - X);
             }
             const R C1 = -(((X + F) * U * U + (X - F) * V * V) * K.c3 / ((R)8 * X) + K.dm3);
             if (s == 0) c[h].x = C1; else c[h].y = C1;

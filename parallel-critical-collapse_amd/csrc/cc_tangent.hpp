@@ -129,11 +129,11 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             const R ddU = half * (a2.x - b2.x), ddV = half * (a2.y - b2.y);
             const int j = t + T * h;
             const R U = rs[RC::U + j], V = rs[RC::V + j], F = rs[RC::F + j];
-            const R tU = rs[RC::DU + j], tV = rs[RC::DV + j];
+            const R tU = rs[RC::DU + j], tV = rs[RC::DV + j], FmX = rs[RC::FMX + j];
             // d Coeff1 (ICG.cpp:834-835 differentiated)
             dC1[h] = -(K.c3 * R(0.125) * inv_x) *
-                     (dF * U * U + R(2.0) * (X + F) * U * dU - dF * V * V + R(2.0) * (X - F) * V * dV);
-            const R inv1 = trcp(two_x * (F + X)), inv2 = trcp(two_x * (F - X));
+                     (dF * U * U + R(2.0) * (X + F) * U * dU - dF * V * V - R(2.0) * FmX * V * dV);
+            const R inv1 = trcp(two_x * (F + X)), inv2 = trcp(two_x * FmX);
             const R dinv1 = -inv1 * inv1 * two_x * dF, dinv2 = -inv2 * inv2 * two_x * dF;
             const R S = U + V, dS = dU + dV;
             const R numU = F * K.dm2 * S - two_x * tU, numV = F * K.dm2 * S + two_x * tV;
@@ -206,10 +206,10 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             const R *rs = rec + fin_off;
             const R r = rs[RC::R + j], E = rs[RC::E + j];
             const int jh = t + T * ph;
-            const R U = rs[RC::U + jh], V = rs[RC::V + jh], F = rs[RC::F + jh];
+            const R U = rs[RC::U + jh], V = rs[RC::V + jh], F = rs[RC::F + jh], FmX = rs[RC::FMX + jh];
             const R X = xs[s], two_x = R(2.0) * X;
             const R BU = K.two_dm3 * F * U * trcp(two_x * (F + X));
-            const R BV = K.two_dm3 * F * V * trcp(two_x * (F - X));
+            const R BV = K.two_dm3 * F * V * trcp(two_x * FmX);
             const R CF = K.dm3 * F * trcp(X);
             const R dIA2 = (s ? c[p].y : c[p].x) * trcp(E);
             const R dr = -r * r * dIA2;

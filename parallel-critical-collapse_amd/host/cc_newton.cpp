@@ -76,6 +76,7 @@ int main(int argc, char *argv[])
         if (!std::filesystem::exists(input)) throw std::invalid_argument("Invalid simulation input path!");
         cchost::SimulationConfig config = cchost::SimulationConfig::load(input);
         if (ignore_converged) config.Converged = false;
+        opt.debug_dir = std::filesystem::absolute(input).parent_path().string();
         if (dry) {
             std::cout << config.to_json().dump(4) << std::endl;
             return 0;

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
@@ -168,6 +169,23 @@ void NewtonSolver::unpack(const std::vector<double> &x)
     fc_.assign(fields.begin() + 2 * N_, fields.end());
 }
 
+// NewtonSolver.cpp:337-341 with OutputWriter::writeMatrix (OutputWriter.cpp:57-74):
+// one row per line, ", " separated, 16 significant digits.
+void NewtonSolver::write_jacobian_csv(size_t iteration) const
+{
+    std::vector<double> J((size_t)n_ * n_);
+    check(cc_last_jacobian(ctx_, J.data()), "cc_last_jacobian");
+    const std::string path = opt_.debug_dir + "/jacobian_" + std::to_string(cfg_.Dim) + "_" +
+                             std::to_string(iteration) + ".csv";
+    std::ofstream f(path);
+    if (!f) throw std::runtime_error("Could not open file for writing: " + path);
+    f << std::setprecision(16);
+    for (int i = 0; i < n_; ++i) {
+        for (int j = 0; j < n_ - 1; ++j) f << J[(size_t)i * n_ + j] << ", ";
+        f << J[(size_t)i * n_ + n_ - 1] << "\n";
+    }
+}
+
 // NewtonSolver.cpp:763-768
 double NewtonSolver::l2norm(const std::vector<double> &v)
 {
@@ -250,6 +268,7 @@ Json NewtonSolver::run(Json *bench)
             say("Newton iteration: " + std::to_string(its + 1));
             errOld = err;
             check(cc_newton_step(ctx_, x_.data(), F0.data(), dx.data()), "Newton step");
+            if (cfg_.Debug) write_jacobian_csv(its + 1);
             err = l2norm(F0);
             history_.push_back(err);
             if (opt_.verbose) std::cout << "Mismatch norm: " << err << std::endl;

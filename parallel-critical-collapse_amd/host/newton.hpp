@@ -45,6 +45,7 @@ struct SolverOptions {
     bool use_graph = false;    // replay each Newton step from a captured hipGraph
     bool reuse_sides = false;  // FP64: reuse the base shoot's unchanged side
     bool verbose = true;       // the reference's progress lines on stdout
+    std::string debug_dir = "."; // where Debug runs write jacobian_<Dim>_<i>.csv
 };
 
 struct SolverError : std::runtime_error {
@@ -87,6 +88,7 @@ private:
     std::vector<double> pack() const;
     void unpack(const std::vector<double> &x);
     void write_final_output(size_t newton_its, double mismatch);
+    void write_jacobian_csv(size_t iteration) const;
     static double l2norm(const std::vector<double> &v);
 };
 

@@ -91,3 +91,24 @@ def test_cc_newton_already_converged(anchored_config, tmp_path):
     F = ctx.residual(x0)
     assert res["IterNewton"] == 0
     assert res["mismatchNorm"] == rms(F)
+
+
+def test_cc_newton_debug_writes_jacobians(anchored_config, tmp_path):
+    """Debug runs write jacobian_<Dim>_<i>.csv per Newton iteration
+    (NewtonSolver.cpp:337-341, OutputWriter.cpp:57-74): the first one is the
+    Jacobian of the step at the initial point."""
+    shape, p, d, x0 = anchored_config
+    d2 = dict(d, Debug=True)
+    p2 = tmp_path / "dbg.json"
+    p2.write_text(json.dumps(d2))
+    r = subprocess.run([CLI, "-i", str(p2), "-o", str(tmp_path / "res.json"), "--quiet"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    path = tmp_path / "jacobian_4.000000_1.csv"
+    assert path.exists()
+    J = np.loadtxt(path, delimiter=",")
+    ctx = cc.Context(cc.Problem.from_shape(shape, precision=cc.CC_MIXED))
+    ctx.newton_step(x0)
+    J_ref = ctx.last_jacobian()
+    assert J.shape == J_ref.shape
+    assert np.max(np.abs(J - J_ref)) <= 1e-15 * np.max(np.abs(J_ref)) * 10   # 16 significant digits in the file
