@@ -291,6 +291,8 @@ def main():
     ap.add_argument("--policies", default="mixed,fp64_tangent,fp64")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-its", type=int, default=12)
+    ap.add_argument("--scale", type=float, default=None,
+                    help="--newton: perturbation scale of the seeded input instead of the workload's")
     ap.add_argument("--save-failures", default="", metavar="DIR",
                     help="--newton: save the Newton vector at which a policy failed to DIR")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -303,6 +305,9 @@ def main():
         cc = importlib.import_module("parallel-critical-collapse_amd")
         W = importlib.import_module("parallel-critical-collapse_amd.workload")
         shape = W.config(args.config)
+        if args.scale is not None:
+            from dataclasses import replace
+            shape = replace(shape, scale=args.scale)
         res = newton_study(cc, shape, args.policies.split(","), args.tol, args.max_its, not args.no_tune,
                            args.save_failures or None)
         print(json.dumps({"workload": f"newton Ntau={shape.Ntau} x_points={shape.X} n={shape.n} Dim={shape.Dim} "

@@ -245,6 +245,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
     constexpr int P = N / T, H = P / 2, M = N / 2;
     __shared__ cx<R> xb[(1 + tangent_db<N, R>()) * 2 * N];
     __shared__ cx<R> ys[2][M];
+    // stage derivatives of the previous step (predictor); at N = 2048 the
+    // array would cost a third of the workgroups per CU, so it is left out
+    constexpr bool PRED = N <= 1024;
+    __shared__ cx<R> fs[2][PRED ? M : 1];
     __shared__ R bc[8];
     __shared__ R red[2 * ((T / 64) > 0 ? T / 64 : 1)];
     const int t = threadIdx.x;
@@ -267,6 +271,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
     int64_t its_sum = 0;
     int32_t st = 0;
     cx<R> dk[2][H];
+    bool have_prev = false;
     for (int step = s_begin; step < s_end; ++step, idx += dir) {
         const double xin = grid[idx], xout = grid[idx + dir];
         const double dxd = xout - xin;
@@ -275,9 +280,25 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
                     A11 = (R)(dxd * a.a11);
         const R B0 = (R)(dxd * a.b0), B1 = (R)(dxd * a.b1);
         const R *rec = a.rec + (size_t)(a.rec_base[ch] + step) * Rec<N>::STEP;
+        // Initial stage iterates: the step-start state, or (after the first
+        // step of a launch) the stages the previous step's derivatives predict,
+        // K_s = y + dx sum_j a_sj f_j(prev).  The linear fixed point then starts
+        // O(dx^2) instead of O(dx) from its solution (the base chains keep the
+        // reference's start, ODEStepper.cpp:193-199).
 #pragma unroll
-        for (int h = 0; h < H; ++h) dk[0][h] = dk[1][h] = ys[0][t + T * h];
-        if (t == T - 2) bc[0] = bc[1] = dk[0][H - 1].x;
+        for (int h = 0; h < H; ++h) {
+            const cx<R> y = ys[0][t + T * h];
+            dk[0][h] = dk[1][h] = y;
+            if (PRED && have_prev) {
+                const cx<R> f0 = fs[0][t + T * h], f1 = fs[1][t + T * h];
+                dk[0][h] = {y.x + A00 * f0.x + A01 * f1.x, y.y + A00 * f0.y + A01 * f1.y};
+                dk[1][h] = {y.x + A10 * f0.x + A11 * f1.x, y.y + A10 * f0.y + A11 * f1.y};
+            }
+        }
+        if (t == T - 2) {
+            bc[0] = dk[0][H - 1].x;
+            bc[1] = dk[1][H - 1].x;
+        }
         __syncthreads();
         bool conv = false;
         int its = 0;
@@ -318,6 +339,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
             // (an overflowed iterate, m2 = inf, is never accepted)
             if (isfinite(m2) && d2 <= (R)(rtol * rtol) * m2) {
                 conv = true;
+                if constexpr (PRED) {
+#pragma unroll
+                    for (int h = 0; h < H; ++h) {
+                        fs[0][t + T * h] = df[0][h];
+                        fs[1][t + T * h] = df[1][h];
+                    }
+                }
                 break;
             }
         }
@@ -327,6 +355,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
             if (t == 0) a.fail_idx[ch] = idx;
             break;
         }
+        have_prev = true;
 #pragma unroll
         for (int h = 0; h < H; ++h) ys[0][t + T * h] = ys[1][t + T * h];
     }
