@@ -152,7 +152,7 @@ def newton_study(cc, shape, policies, tol, max_its, tune, save_failures=None):
     for name in policies:
         ctx = cc.Context(cc.Problem.from_shape(shape, precision=names[name]))
         x = W.initial_x(shape)
-        if tune:
+        if tune and name != "fp64":   # FP64 finite differences keep the default schedule
             ctx.autotune(x)
         hist, secs = [], []
         err, err_old, increase, reached, failure = 1.0, 1.0, 0, None, None
@@ -184,10 +184,10 @@ def newton_study(cc, shape, policies, tol, max_its, tune, save_failures=None):
                      "s_iterations": [round(v, 4) for v in secs], "total_s": float(np.sum(secs)),
                      "failure": failure, "schedule": ctx.schedule().as_dict()}
         ctx.close()
-    ref = "fp64_tangent" if "fp64_tangent" in out else None
-    if ref:
-        for name, r in out.items():
-            r["Delta_rel_diff_vs_fp64_tangent"] = abs(r["Delta"] - out[ref]["Delta"]) / abs(out[ref]["Delta"])
+    for ref in ("fp64_tangent", "fp64"):
+        if ref in out and out[ref]["iterations_to_tol"] is not None:
+            for name, r in out.items():
+                r[f"Delta_rel_diff_vs_{ref}"] = abs(r["Delta"] - out[ref]["Delta"]) / abs(out[ref]["Delta"])
     return out
 
 

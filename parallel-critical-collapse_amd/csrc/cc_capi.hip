@@ -95,7 +95,7 @@ struct cc_ctx {
     double *J = nullptr;                 // n x n
     double *A = nullptr;                 // n x n LU work (FP64)
     float *A32 = nullptr;                // n x n LU work (FP32)
-    int32_t *ipiv = nullptr, *info = nullptr;
+    int32_t *perm = nullptr, *lu_swap = nullptr, *info = nullptr;   // LU row permutation, panel swap list
     ColumnBuffers main;                  // base + every column (FP64 Jacobian)
     ColumnBuffers base;                  // base + Delta column (residual, mixed policy)
     Plan main_plan, base_plan;
@@ -104,6 +104,7 @@ struct cc_ctx {
     int64_t *rec_base_b = nullptr;       // record slots of the base launch's chains
     int64_t *rec_base_t = nullptr;       // record slots of the tangent chains
     void *dY = nullptr;                  // [n-1][M] tangent states (cx<float>, or cx<double>)
+    void *fprev = nullptr;               // [n-1][2][M] tangent predictor state across launches
     int32_t *t_start = nullptr, *t_nsteps = nullptr, *t_dir = nullptr, *t_status = nullptr, *t_fail = nullptr;
     int32_t *t_cols = nullptr, *t_lcols = nullptr, *t_rcols = nullptr, *t_lch = nullptr, *t_rch = nullptr;
     int32_t *t_lidx = nullptr, *t_ridx = nullptr;
@@ -147,7 +148,7 @@ struct BatchClass {
     int32_t *start = nullptr, *nsteps = nullptr, *dir = nullptr, *status = nullptr, *fail = nullptr, *goff = nullptr;
     int64_t *its = nullptr, *rec_base = nullptr;
     // tangent launch
-    void *dY = nullptr, *rec = nullptr;
+    void *dY = nullptr, *rec = nullptr, *fprev = nullptr;
     double *t_dim = nullptr;
     int32_t *t_start = nullptr, *t_nsteps = nullptr, *t_dir = nullptr, *t_status = nullptr, *t_fail = nullptr,
             *t_goff = nullptr, *t_bs = nullptr;
@@ -463,14 +464,25 @@ cc_status eval_columns(cc_ctx *c, const Plan &P, Counts k, bool is_base, int32_t
 template <typename R>
 cc_status lu_factor(cc_ctx *c, R *dA, int n, hipStream_t s)
 {
-    for (int k0 = 0; k0 < n; k0 += ccd::LU_NB) {
-        hipLaunchKernelGGL((ccd::k_lu_panel<R>), dim3(1), dim3(ccd::LU_PANEL_T), 0, s, dA, n, k0, c->ipiv, c->info);
-        const int nb = std::min(ccd::LU_NB, n - k0);
-        const int rest = n - k0 - nb;
+    constexpr int NB = ccd::lu_nb<R>();
+    if (n > ccd::LU_MAXN) {
+        c->err = "LU order above " + std::to_string(ccd::LU_MAXN);
+        return CC_BADARG;
+    }
+    const unsigned colblocks = (unsigned)((n + 63) / 64);
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int m = n - k0, nb = std::min(NB, m), rest = m - nb;
+        if (m <= ccd::LU_PANEL_ROWS)
+            hipLaunchKernelGGL((ccd::k_lu_panel<R, true>), dim3(1), dim3(ccd::LU_PANEL_T), 0, s, dA, n, k0, c->perm,
+                               c->lu_swap, c->info);
+        else
+            hipLaunchKernelGGL((ccd::k_lu_panel<R, false>), dim3(1), dim3(ccd::LU_PANEL_T), 0, s, dA, n, k0, c->perm,
+                               c->lu_swap, c->info);
+        hipLaunchKernelGGL((ccd::k_lu_laswp<R>), dim3(colblocks), dim3(256), 0, s, dA, n, k0, nb, c->lu_swap);
         if (rest > 0) {
-            hipLaunchKernelGGL((ccd::k_lu_trsm<R>), dim3((rest + 127) / 128), dim3(128), 0, s, dA, n, k0);
+            hipLaunchKernelGGL((ccd::k_lu_trsm<R>), dim3((rest + 255) / 256), dim3(256), 0, s, dA, n, k0);
             const int tiles = (rest + 63) / 64;
-            hipLaunchKernelGGL((ccd::k_lu_gemm<R>), dim3(tiles, tiles), dim3(256), 0, s, dA, n, k0);
+            hipLaunchKernelGGL((ccd::k_lu_gemm<R>), dim3(tiles, tiles), dim3(256), 0, s, dA, n, k0, nb);
         }
     }
     HIPCHK(c, hipGetLastError());
@@ -488,15 +500,15 @@ cc_status enqueue_solve(cc_ctx *c, int n, const double *d_J, const double *d_rhs
         HIPCHK(c, hipMemcpyAsync(c->A, d_J, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
         CCCHK(lu_factor<double>(c, c->A, n, s));
         HIPCHK(c, hipMemcpyAsync(d_sol, d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL((ccd::k_lu_solve<double, double>), dim3(1), dim3(1024), 0, s, c->A, n, c->ipiv, d_sol);
+        hipLaunchKernelGGL((ccd::k_lu_solve<double, double>), dim3(1), dim3(1024), 0, s, c->A, n, c->perm, d_sol);
     } else {
         hipLaunchKernelGGL((ccd::k_cast_to<float>), dim3(g), dim3(256), 0, s, d_J, c->A32, nn);
         CCCHK(lu_factor<float>(c, c->A32, n, s));
         HIPCHK(c, hipMemcpyAsync(d_sol, d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL((ccd::k_lu_solve<float, double>), dim3(1), dim3(1024), 0, s, c->A32, n, c->ipiv, d_sol);
+        hipLaunchKernelGGL((ccd::k_lu_solve<float, double>), dim3(1), dim3(1024), 0, s, c->A32, n, c->perm, d_sol);
         for (int it = 0; it < c->p.refine_iters; ++it) {       // FP64 iterative refinement
             hipLaunchKernelGGL(ccd::k_residual, dim3(n), dim3(256), 0, s, d_J, d_sol, d_rhs, c->r, n);
-            hipLaunchKernelGGL((ccd::k_lu_solve<float, double>), dim3(1), dim3(1024), 0, s, c->A32, n, c->ipiv,
+            hipLaunchKernelGGL((ccd::k_lu_solve<float, double>), dim3(1), dim3(1024), 0, s, c->A32, n, c->perm,
                                c->r);
             hipLaunchKernelGGL(ccd::k_axpy, dim3((n + 255) / 256), dim3(256), 0, s, d_sol, c->r, n);
         }
@@ -534,6 +546,7 @@ ccd::TangentParams<R> tangent_params(const cc_ctx *c)
 {
     ccd::TangentParams<R> a{};
     a.dY = static_cast<cx<R> *>(c->dY);
+    a.fprev = static_cast<cx<R> *>(c->fprev);
     a.rec = static_cast<const R *>(c->rec);
     a.rec_base = c->rec_base_t;
     a.start = c->t_start;
@@ -1010,6 +1023,7 @@ cc_status batch_enqueue_class_n(cc_batch *b, BatchClass &C)
     ab.rec_base = C.rec_base;
     ccd::TangentParams<R> at = tangent_params<N, R>(c0);
     at.dY = static_cast<cx<R> *>(C.dY);
+    at.fprev = static_cast<cx<R> *>(C.fprev);
     at.rec = static_cast<const R *>(C.rec);
     at.rec_base = C.t_rec_base;
     at.start = C.t_start;
@@ -1246,7 +1260,8 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
     ALLOC(c->J, (size_t)n * n);
     ALLOC(c->A, (size_t)n * n);
     ALLOC(c->A32, (size_t)n * n);
-    ALLOC(c->ipiv, n);
+    ALLOC(c->perm, n);
+    ALLOC(c->lu_swap, 1 + 2 * ccd::LU_SWAP_MAX);
     ALLOC(c->info, 1);
     for (ColumnBuffers *B : {&c->main, &c->base}) {
         const int mc = B == &c->main ? c->maxcols : 2, mch = B == &c->main ? c->maxchains : 4;
@@ -1264,11 +1279,13 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
     if (own_tangent) {
         // record and tangent states in the tangent's precision (Rec<N>::STEP values per step)
         const size_t elem = p->precision == CC_FP64_TANGENT ? sizeof(double) : sizeof(float);
-        char *recp = nullptr, *dyp = nullptr;
+        char *recp = nullptr, *dyp = nullptr, *fpp = nullptr;
         ALLOC(recp, (size_t)c->X * (size_t)rec_step_values(N) * elem);
         ALLOC(dyp, (size_t)n * M * 2 * elem);
+        ALLOC(fpp, (size_t)n * 2 * M * 2 * elem);
         c->rec = recp;
         c->dY = dyp;
+        c->fprev = fpp;
     }
     ALLOC(c->rec_base_b, 4);
     ALLOC(c->rec_base_t, n);
@@ -1890,7 +1907,7 @@ cc_status cc_batch_create(const cc_problem *problems, int32_t count, cc_batch **
         }
         C.nbase = 4 * (int)C.members.size();
         hipError_t e2 = hipSuccess;
-        char *dyp = nullptr, *recp = nullptr;
+        char *dyp = nullptr, *recp = nullptr, *fpp = nullptr;
 #define BALLOC(ptr, cnt) \
     if (e2 == hipSuccess) e2 = balloc(b, &(ptr), (cnt));
         BALLOC(C.Y, (size_t)C.nbase * M);
@@ -1901,6 +1918,7 @@ cc_status cc_batch_create(const cc_problem *problems, int32_t count, cc_batch **
         BALLOC(C.its, C.nbase);
         BALLOC(C.rec_base, C.nbase);
         BALLOC(dyp, (size_t)C.ntan * M * 2 * elem);
+        BALLOC(fpp, (size_t)C.ntan * 2 * M * 2 * elem);
         BALLOC(recp, nrec * step * elem);
         BALLOC(C.t_dim, C.ntan);
         for (int32_t **q : {&C.t_start, &C.t_nsteps, &C.t_dir, &C.t_status, &C.t_fail, &C.t_goff, &C.t_bs})
@@ -1911,6 +1929,7 @@ cc_status cc_batch_create(const cc_problem *problems, int32_t count, cc_batch **
         if (bfail(b, "hipMalloc", e2)) return bail(CC_DEVICE);
         C.dY = dyp;
         C.rec = recp;
+        C.fprev = fpp;
         // point every member into the class arrays; per-chain tables for the shared launches
         std::vector<double> dim_b, dim_t;
         std::vector<int32_t> goff_b, goff_t, bs_t, tstart, tnsteps, tdir;
@@ -1934,6 +1953,7 @@ cc_status cc_batch_create(const cc_problem *problems, int32_t count, cc_batch **
             if (bfail(b, "plan", upload_plan(c->base_plan, c->base))) return bail(CC_DEVICE);
             c->rec = recp + ro * step * elem;
             c->dY = dyp + (size_t)to * M * 2 * elem;
+            c->fprev = fpp + (size_t)to * 2 * M * 2 * elem;
             c->t_status = C.t_status + to;
             c->t_fail = C.t_fail + to;
             c->t_its = C.t_its + to;

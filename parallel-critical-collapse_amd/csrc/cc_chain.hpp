@@ -87,7 +87,7 @@ struct RhsConst {
 // Evaluate f_s = RHS(yk_s, xs_s) for both stages (ODEStepper.cpp:92-136).
 // bc[0], bc[1]: Re of mode M-2 of each stage state (the mode-2 fix value),
 // published by the caller before the call.
-template <int N, int T, typename R, typename RT = float>
+template <int N, int T, typename R, typename RT = float, int DB = 0>
 __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> (&f)[2][N / T / 2],
                                          const R (&xs)[2], const RhsConst<R> &K, const R *__restrict__ invm,
                                          cx<R> *xb, R *bc, const cx<R> *__restrict__ tw, int t,
@@ -136,7 +136,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
             z[1][p] = {w * v.y, -(w * v.x)};
         }
         // (2) back to tau (804-805)
-        fft_block<N, T, -1, 2>(z, xb, tw, t);
+        fft_block<N, T, -1, 2, R, DB>(z, xb, tw, t);
         // (3) half-period split (814-827), IA2 coefficients (834-836), RHS products
         const R X = xs[s];
         const R inv_x = (R)1 / X;
@@ -172,7 +172,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     for (int h = 0; h < H; ++h) c[h + H] = c[h];
 
     // (4) solveInhom step 1 for both stages: F' = Coeff1 (264-270)
-    fft1<N, T, +1>(c, xb, tw, t);
+    fft1<N, T, +1, R, DB>(c, xb, tw, t);
 #pragma unroll
     for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
     if (t == 0) {                       // lambda_s = mean of Coeff1_s
@@ -185,7 +185,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         const cx<R> v = c[p];
         c[p] = {-(v.y * im), v.x * im};
     }
-    fft1<N, T, -1>(c, xb, tw, t);
+    fft1<N, T, -1, R, DB>(c, xb, tw, t);
     // (5) h = -g e^{F} (276-279), forward (283); keep e^{F}
     R E[2][P];
 #pragma unroll
@@ -194,7 +194,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         E[1][p] = exp(c[p].y);
         c[p] = {K.mg * E[0][p], K.mg * E[1][p]};
     }
-    fft1<N, T, +1>(c, xb, tw, t);
+    fft1<N, T, +1, R, DB>(c, xb, tw, t);
 #pragma unroll
     for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
     // (6) separate the two Hermitian spectra, lamIntegrate with lambda_s (284)
@@ -240,7 +240,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         }
         c[p] = {G[0].x - G[1].y, G[0].y + G[1].x};
     }
-    fft1<N, T, -1>(c, xb, tw, t);
+    fft1<N, T, -1, R, DB>(c, xb, tw, t);
 
     // (7) pointwise RHS with r = 1/IA2 = e^{F}/X (288-291; ODEStepper.cpp:109-122)
     cx<R> yp[2][P];
@@ -261,7 +261,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         }
     }
     // (8) FieldsToStateVector (766-778) and Re dY[2] = 0 (ODEStepper.cpp:128)
-    fft_block<N, T, +1, 2>(yp, xb, tw, t);
+    fft_block<N, T, +1, 2, R, DB>(yp, xb, tw, t);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -297,12 +297,15 @@ template <int N, int T, typename R>
 constexpr int chain_waves() { return sizeof(R) == 8 ? (N / T >= 8 ? 1 : 2) : 3; }
 
 // RT: element type of the base-trajectory record (float for the FP32 tangent,
-// double for the FP64 one); unused when a.rec is null.
-template <int N, int T, typename R, typename RT = float>
+// double for the FP64 one); unused when a.rec is null.  DB = 1 double-buffers
+// the transform exchanges (one barrier each instead of two) at twice the LDS:
+// for the few latency-bound base chains of a tangent policy, not for the
+// chip-filling FP64 Jacobian launches that need two workgroups per CU.
+template <int N, int T, typename R, typename RT = float, int DB = 0>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N, T, R>()))) void k_chain(ChainParams<R> a)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
-    __shared__ cx<R> xb[2 * N];
+    __shared__ cx<R> xb[(1 + DB) * 2 * N];
     __shared__ R bc[8];
     __shared__ R red[(T / 64) > 0 ? T / 64 : 1];
     const int t = threadIdx.x;
@@ -345,7 +348,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
             rec = static_cast<RT *>(a.rec) + (size_t)(a.rec_base[ch] + step) * Rec<N>::STEP;
         for (its = 0; its < a.maxIts; ++its) {
             cx<R> f[2][H];
-            rhs_pair<N, T, R, RT>(yk, f, xs, K, a.invm, xb, bc, a.tw, t, rec);
+            rhs_pair<N, T, R, RT, DB>(yk, f, xs, K, a.invm, xb, bc, a.tw, t, rec);
             R loc = (R)0;
 #pragma unroll
             for (int h = 0; h < H; ++h) {

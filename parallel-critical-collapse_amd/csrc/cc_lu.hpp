@@ -1,36 +1,99 @@
 // cc_lu.hpp -- dense LU with partial pivoting and the triangular solves
 // (the device replacement of LAPACKE_dgesv, NewtonSolver.cpp:739-757).
 //
-// Right-looking blocked LU on a row-major n x n matrix, panel width NB:
-//   k_lu_panel  one workgroup factors the NB-wide panel (pivot search by a
-//               workgroup max-reduction, first index on ties as LAPACK's
-//               idamax; full-row swaps; rank-1 updates inside the panel)
-//   k_lu_trsm   U12 = L11^{-1} A12 (unit lower triangular, one column per thread)
+// Right-looking blocked LU on a row-major n x n matrix, panel width NB (32 in
+// FP32, 16 in FP64):
+//   k_lu_panel  one workgroup factors the n-k0 x NB panel, held in LDS when it
+//               has at most LU_PANEL_ROWS rows (else worked in global memory):
+//               pivot = first row index of the largest |a| as LAPACK's idamax,
+//               swaps inside the panel, rank-1 updates.  It writes the panel's
+//               net row permutation as a swap list and composes it into the
+//               factorization's row permutation perm (P b = b[perm]).
+//   k_lu_laswp  applies the swap list to every column outside the panel (so
+//               the stored L carries the row interchanges as dgetrf's does)
+//   k_lu_trsm   U12 = L11^{-1} A12 (unit lower triangular, one column per
+//               thread, the column in registers)
 //   k_lu_gemm   A22 -= L21 U12, 64x64 tiles per workgroup.  FP64 tiles run on
 //               v_mfma_f64_16x16x4f64, FP32 tiles on v_mfma_f32_16x16x4f32.
-//   k_lu_solve  forward/back substitution for one right-hand side
+//   k_lu_solve  P b, then blocked forward/back substitution for one
+//               right-hand side (64-row blocks: the off-diagonal part as
+//               wave dot products, the diagonal block by one wave)
 #pragma once
 #include "cc_device.hpp"
 
 namespace ccd {
 
-constexpr int LU_NB = 64;
-constexpr int LU_PANEL_T = 256;
+constexpr int LU_MAXN = 2048;         // largest order (3/4 of the largest Ntau, rounded up)
+constexpr int LU_PANEL_ROWS = 1024;   // panels with more rows are factored in global memory
+constexpr int LU_PANEL_T = 512;
+constexpr int LU_SWAP_MAX = 64;       // rows one panel can move: 2 x the widest panel
+template <typename R>
+constexpr int lu_nb()
+{
+    return sizeof(R) == 8 ? 16 : 32;
+}
 
 template <typename R>
-__global__ __launch_bounds__(LU_PANEL_T) void k_lu_panel(R *A, int n, int k0, int32_t *ipiv, int32_t *info)
+__device__ __forceinline__ R wave_sum(R v)
 {
-    __shared__ R sval[LU_PANEL_T / 64];
-    __shared__ int sidx[LU_PANEL_T / 64];
-    __shared__ int spiv;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Value of x in lane l (l uniform across the wave).
+__device__ __forceinline__ float lane_value(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double lane_value(double x, int l)
+{
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)v, l);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Factor columns [k0, k0+w) over rows [k0, n).  perm: row permutation so far
+// (identity before the first panel, read as such when k0 == 0); swap:
+// [count, (dst, src) x count] -- after the panel, row dst holds what row src
+// held before it.
+template <typename R, bool INLDS>
+__global__ __launch_bounds__(LU_PANEL_T) void k_lu_panel(R *A, int n, int k0, int32_t *perm, int32_t *swap,
+                                                          int32_t *info)
+{
+    constexpr int NB = lu_nb<R>(), T = LU_PANEL_T, W = T / 64;
+    constexpr int LD = NB + 1;
+    constexpr int MAXR = LU_MAXN / T;
+    __shared__ R Ps[INLDS ? LU_PANEL_ROWS * LD : 1];
+    __shared__ int rowid[LU_MAXN];
+    __shared__ R sval[W];
+    __shared__ int sidx[W];
+    __shared__ int spiv, scount;
     const int t = threadIdx.x;
-    const int kend = min(k0 + LU_NB, n);
-    for (int j = k0; j < kend; ++j) {
-        // pivot: first row index of max |A[i][j]|, i >= j
+    const int m = n - k0, w = min(NB, m);
+    R *P;
+    int ld;                              // n <= LU_MAXN: every offset fits an int
+    if constexpr (INLDS) {
+        P = Ps;
+        ld = LD;
+        for (int e = t; e < m * w; e += T) {
+            const int i = e / w, c = e - i * w;
+            Ps[i * LD + c] = A[(size_t)(k0 + i) * n + k0 + c];
+        }
+    } else {
+        P = A + (size_t)k0 * n + k0;
+        ld = n;
+    }
+    for (int i = t; i < m; i += T) rowid[i] = i;
+    if (t == 0) scount = 0;
+    __syncthreads();
+    for (int j = 0; j < w; ++j) {
+        // pivot: first row index of max |P[i][j]|, i >= j
         R best = (R)-1;
-        int bi = n;
-        for (int i = j + t; i < n; i += LU_PANEL_T) {
-            const R v = fabs(A[(size_t)i * n + j]);
+        int bi = m;
+        for (int i = j + t; i < m; i += T) {
+            const R v = fabs(P[i * ld + j]);
             if (v > best) {
                 best = v;
                 bi = i;
@@ -53,59 +116,119 @@ __global__ __launch_bounds__(LU_PANEL_T) void k_lu_panel(R *A, int n, int k0, in
         if (t == 0) {
             R b = sval[0];
             int i0 = sidx[0];
-            for (int w = 1; w < LU_PANEL_T / 64; ++w)
-                if (sval[w] > b || (sval[w] == b && sidx[w] < i0)) {
-                    b = sval[w];
-                    i0 = sidx[w];
+            for (int v = 1; v < W; ++v)
+                if (sval[v] > b || (sval[v] == b && sidx[v] < i0)) {
+                    b = sval[v];
+                    i0 = sidx[v];
                 }
             // a column with no comparable entry (all NaN) yields no candidate:
-            // keep the diagonal so every row index stays in [j, n)
-            if (i0 >= n) i0 = j;
+            // keep the diagonal so every row index stays in [j, m)
+            if (i0 >= m) i0 = j;
             spiv = i0;
-            ipiv[j] = i0 + 1;
             // zero pivot as LAPACK; a NaN pivot is reported the same way
-            if (!(fabs(A[(size_t)i0 * n + j]) > (R)0) && info[0] == 0) info[0] = j + 1;
+            if (!(fabs(P[i0 * ld + j]) > (R)0) && info[0] == 0) info[0] = k0 + j + 1;
+            if (i0 != j) {
+                const int r = rowid[j];
+                rowid[j] = rowid[i0];
+                rowid[i0] = r;
+            }
         }
         __syncthreads();
         const int piv = spiv;
-        if (piv != j) {
-            for (int c = t; c < n; c += LU_PANEL_T) {
-                const R tmp = A[(size_t)j * n + c];
-                A[(size_t)j * n + c] = A[(size_t)piv * n + c];
-                A[(size_t)piv * n + c] = tmp;
-            }
+        if (piv != j && t < w) {
+            const R tmp = P[j * ld + t];
+            P[j * ld + t] = P[piv * ld + t];
+            P[piv * ld + t] = tmp;
         }
         __syncthreads();
-        const R pv = A[(size_t)j * n + j];
-        if (pv != (R)0) {
-            const R inv = (R)1 / pv;
-            for (int i = j + 1 + t; i < n; i += LU_PANEL_T) A[(size_t)i * n + j] *= inv;
-        }
-        __syncthreads();
-        // rank-1 update of the rest of the panel
-        const int w = kend - (j + 1);
-        if (w > 0) {
-            for (int e = t; e < (n - j - 1) * w; e += LU_PANEL_T) {
-                const int i = j + 1 + e / w, c = j + 1 + e % w;
-                A[(size_t)i * n + c] -= A[(size_t)i * n + j] * A[(size_t)j * n + c];
+        // scale column j below the diagonal, then the rank-1 update of the
+        // rest of the row (each row by one thread: no barrier in between)
+        const R pv = P[j * ld + j];
+        const bool scale = pv != (R)0;
+        const R inv = scale ? (R)1 / pv : (R)0;
+        for (int i = j + 1 + t; i < m; i += T) {
+            R lij = P[i * ld + j];
+            if (scale) {
+                lij *= inv;
+                P[i * ld + j] = lij;
             }
+            for (int c = j + 1; c < w; ++c) P[i * ld + c] -= lij * P[j * ld + c];
         }
         __syncthreads();
     }
+    if constexpr (INLDS) {
+        for (int e = t; e < m * w; e += T) {
+            const int i = e / w, c = e - i * w;
+            A[(size_t)(k0 + i) * n + k0 + c] = Ps[i * LD + c];
+        }
+    }
+    // compose the panel's permutation into perm; list the rows that moved
+    int pnew[MAXR];
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) {
+        const int r = t + T * q;
+        if (r < m) pnew[q] = k0 == 0 ? rowid[r] : perm[k0 + rowid[r]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) {
+        const int r = t + T * q;
+        if (r < m) {
+            perm[k0 + r] = pnew[q];
+            if (rowid[r] != r) {
+                const int e = atomicAdd(&scount, 1);
+                swap[1 + 2 * e] = k0 + r;
+                swap[2 + 2 * e] = k0 + rowid[r];
+            }
+        }
+    }
+    __syncthreads();
+    if (t == 0) swap[0] = scount;
 }
 
-// U12 = L11^{-1} A12: rows [k0, k0+nb), columns [k0+nb, n)
+// Apply a panel's swap list to the columns outside the panel [k0, k0+w).
+// 64 columns per workgroup; all moved rows are read before any is written.
 template <typename R>
-__global__ void k_lu_trsm(R *A, int n, int k0)
+__global__ __launch_bounds__(256) void k_lu_laswp(R *A, int n, int k0, int w, const int32_t *swap)
 {
-    const int nb = min(LU_NB, n - k0);
-    const int c = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    for (int i = 1; i < nb; ++i) {
-        R s = A[(size_t)(k0 + i) * n + c];
-        for (int l = 0; l < i; ++l) s -= A[(size_t)(k0 + i) * n + k0 + l] * A[(size_t)(k0 + l) * n + c];
-        A[(size_t)(k0 + i) * n + c] = s;
+    __shared__ R vals[LU_SWAP_MAX][64];
+    const int cnt = swap[0];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + lane;
+    const bool col = c < n && (c < k0 || c >= k0 + w);
+    for (int q = wave; q < cnt; q += 4)
+        if (col) vals[q][lane] = A[(size_t)swap[2 + 2 * q] * n + c];
+    __syncthreads();
+    for (int q = wave; q < cnt; q += 4)
+        if (col) A[(size_t)swap[1 + 2 * q] * n + c] = vals[q][lane];
+}
+
+// U12 = L11^{-1} A12: rows [k0, k0+NB), columns [k0+NB, n); the panel is
+// full width whenever columns remain to its right.
+template <typename R>
+__global__ __launch_bounds__(256) void k_lu_trsm(R *A, int n, int k0)
+{
+    constexpr int NB = lu_nb<R>();
+    __shared__ R L[NB][NB];
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+        const int i = e / NB, l = e % NB;
+        L[i][l] = A[(size_t)(k0 + i) * n + k0 + l];
     }
+    __syncthreads();
+    const int c = k0 + NB + blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    R v[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) v[i] = A[(size_t)(k0 + i) * n + c];
+#pragma unroll
+    for (int i = 1; i < NB; ++i) {
+        R s = v[i];
+#pragma unroll
+        for (int l = 0; l < i; ++l) s -= L[i][l] * v[l];
+        v[i] = s;
+    }
+#pragma unroll
+    for (int i = 1; i < NB; ++i) A[(size_t)(k0 + i) * n + c] = v[i];
 }
 
 // MFMA 16x16x4 wrappers.  One wave computes a 16x16 tile D += A(16x4) B(4x16).
@@ -138,9 +261,8 @@ __device__ __forceinline__ void mfma16(float a, float b, float (&acc)[4])
 // Workgroup: 256 threads = 4 waves, tile 64x64, each wave a 32x32 quadrant
 // (2x2 MFMA tiles); K staged through LDS in chunks of 16.
 template <typename R>
-__global__ __launch_bounds__(256) void k_lu_gemm(R *A, int n, int k0)
+__global__ __launch_bounds__(256) void k_lu_gemm(R *A, int n, int k0, int nb)
 {
-    const int nb = min(LU_NB, n - k0);
     const int r0 = k0 + nb + blockIdx.y * 64;
     const int c0 = k0 + nb + blockIdx.x * 64;
     __shared__ R La[64][17];
@@ -191,33 +313,66 @@ __global__ __launch_bounds__(256) void k_lu_gemm(R *A, int n, int k0)
             }
 }
 
-// Solve with the factors: b <- P b, L y = b, U x = y.  One workgroup.
+// Solve with the factors: y = P b (gathered through perm), L y' = y,
+// U x = y'.  One workgroup of 16 waves; b is replaced by x.
 template <typename R, typename RB>
-__global__ __launch_bounds__(1024) void k_lu_solve(const R *A, int n, const int32_t *ipiv, RB *b)
+__global__ __launch_bounds__(1024) void k_lu_solve(const R *A, int n, const int32_t *perm, RB *b)
 {
-    const int t = threadIdx.x;
-    if (t == 0)
-        for (int j = 0; j < n; ++j) {
-            const int p = ipiv[j] - 1;
-            if (p != j) {
-                const RB tmp = b[j];
-                b[j] = b[p];
-                b[p] = tmp;
-            }
-        }
+    __shared__ RB bs[LU_MAXN];
+    __shared__ R D[64][65];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int r = t; r < n; r += 1024) bs[r] = b[perm[r]];
     __syncthreads();
-    for (int i = 1; i < n; ++i) {          // column-oriented forward substitution
-        const RB bi = b[i - 1];
-        for (int r = i + t; r < n; r += blockDim.x) b[r] -= (RB)A[(size_t)r * n + (i - 1)] * bi;
+    // unit lower L, 64-row blocks top-down
+    for (int i0 = 0; i0 < n; i0 += 64) {
+        const int ib = min(64, n - i0);
+        for (int e = t; e < 64 * 64; e += 1024) {
+            const int i = e >> 6, l = e & 63;
+            D[i][l] = (i < ib && l < ib) ? A[(size_t)(i0 + i) * n + i0 + l] : (R)0;
+        }
+        for (int i = wave; i < ib; i += 16) {
+            RB s = (RB)0;
+            for (int l = lane; l < i0; l += 64) s += (RB)A[(size_t)(i0 + i) * n + l] * bs[l];
+            s = wave_sum(s);
+            if (lane == 0) bs[i0 + i] -= s;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            RB x = lane < ib ? bs[i0 + lane] : (RB)0;
+            for (int l = 0; l < ib - 1; ++l) {
+                const RB xl = lane_value(x, l);
+                if (lane > l) x -= (RB)D[lane][l] * xl;
+            }
+            if (lane < ib) bs[i0 + lane] = x;
+        }
         __syncthreads();
     }
-    for (int i = n - 1; i >= 0; --i) {
-        if (t == 0) b[i] = b[i] / (RB)A[(size_t)i * n + i];
+    // upper U, 64-row blocks bottom-up
+    for (int i1 = n; i1 > 0; i1 -= 64) {
+        const int i0 = max(0, i1 - 64), ib = i1 - i0;
+        for (int e = t; e < 64 * 64; e += 1024) {
+            const int i = e >> 6, l = e & 63;
+            D[i][l] = (i < ib && l < ib) ? A[(size_t)(i0 + i) * n + i0 + l] : (R)0;
+        }
+        for (int i = wave; i < ib; i += 16) {
+            RB s = (RB)0;
+            for (int l = i1 + lane; l < n; l += 64) s += (RB)A[(size_t)(i0 + i) * n + l] * bs[l];
+            s = wave_sum(s);
+            if (lane == 0) bs[i0 + i] -= s;
+        }
         __syncthreads();
-        const RB bi = b[i];
-        for (int r = t; r < i; r += blockDim.x) b[r] -= (RB)A[(size_t)r * n + i] * bi;
+        if (wave == 0) {
+            RB x = lane < ib ? bs[i0 + lane] : (RB)0;
+            for (int l = ib - 1; l >= 0; --l) {
+                if (lane == l) x = x / (RB)D[l][l];
+                const RB xl = lane_value(x, l);
+                if (lane < l) x -= (RB)D[lane][l] * xl;
+            }
+            if (lane < ib) bs[i0 + lane] = x;
+        }
         __syncthreads();
     }
+    for (int r = t; r < n; r += 1024) b[r] = bs[r];
 }
 
 // r = b - A x (FP64 residual for iterative refinement), one row per thread block lane-group

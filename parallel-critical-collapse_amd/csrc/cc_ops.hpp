@@ -25,6 +25,7 @@ template <int N>
 struct Ops {
     static constexpr int kThreads = default_threads(N);
     static constexpr int kAltThreads = alt_threads(N);
+    static constexpr int kFewChains = 256;  // one workgroup per CU
     static bool valid_threads(int T) { return T == kThreads || (kAltThreads != 0 && T == kAltThreads); }
 
     // k_boundary: Y in the chain precision (double or float)
@@ -32,7 +33,8 @@ struct Ops {
                                const int32_t *chain, double eps, double Xb, double Dim, const cx<double> *tw, void *Y,
                                double *delta);
     // k_chain; FP32 chains run at kThreads only.  a.rec (if set) is written as
-    // double when rec_fp64 (kThreads only), else as float.
+    // double when rec_fp64 (kThreads only), else as float.  Launches of at most
+    // kFewChains chains at kThreads double-buffer the transform exchanges.
     static hipError_t chain(hipStream_t s, int T, int nchains, const ChainParams<double> &a, bool rec_fp64 = false);
     static hipError_t chain(hipStream_t s, int T, int nchains, const ChainParams<float> &a);
     static hipError_t mismatch(hipStream_t s, bool fp32, int ncols, const void *Y, const int32_t *lidx,

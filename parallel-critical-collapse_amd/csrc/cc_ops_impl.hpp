@@ -35,9 +35,19 @@ hipError_t Ops<N>::boundary(hipStream_t s, bool left, bool fp32, int ncols, cons
 template <int N>
 hipError_t Ops<N>::chain(hipStream_t s, int T, int nchains, const ChainParams<double> &a, bool rec_fp64)
 {
+    // Up to one workgroup per CU, the doubled LDS of the double-buffered
+    // exchanges costs no occupancy: the base chains of the tangent policies and
+    // the residual.  The exchanges move the same values, so both variants give
+    // bit-identical chains.
+    constexpr int DB = N <= 1024 ? 1 : 0;
+    const bool few = nchains <= kFewChains;
     if (rec_fp64) {
         if (T != kThreads) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_chain<N, kThreads, double, double>), dim3((unsigned)nchains), dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((k_chain<N, kThreads, double, double, DB>), dim3((unsigned)nchains), dim3(kThreads), 0, s,
+                           a);
+    } else if (few && T == kThreads) {
+        hipLaunchKernelGGL((k_chain<N, kThreads, double, float, DB>), dim3((unsigned)nchains), dim3(kThreads), 0, s,
+                           a);
     } else if (T == kThreads) {
         hipLaunchKernelGGL((k_chain<N, kThreads, double>), dim3((unsigned)nchains), dim3(kThreads), 0, s, a);
     } else {

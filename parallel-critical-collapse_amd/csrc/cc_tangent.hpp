@@ -32,6 +32,8 @@ namespace ccd {
 template <typename R>
 struct TangentParams {
     cx<R> *dY;            // [nchains][M] tangent states, in/out
+    cx<R> *fprev;         // [nchains][2][M] stage derivatives of a chain's last step (the
+                          // predictor's state across launches), in/out; may be null
     const R *rec;         // base record (same precision as the tangent)
     const int64_t *rec_base;  // per chain: record slot of its step 0 (the base chain of its side)
     const int32_t *start, *nsteps, *dir;
@@ -271,7 +273,18 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
     int64_t its_sum = 0;
     int32_t st = 0;
     cx<R> dk[2][H];
-    bool have_prev = false;
+    // the predictor continues across launches: a chain resumed mid-integration
+    // reloads its last step's derivatives (the result does not depend on
+    // where the step sequence is cut into launches)
+    bool have_prev = PRED && a.fprev && s_begin > 0;
+    if (have_prev) {
+        const cx<R> *fp = a.fprev + (size_t)ch * 2 * M;
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            fs[0][t + T * h] = fp[t + T * h];
+            fs[1][t + T * h] = fp[M + t + T * h];
+        }
+    }
     for (int step = s_begin; step < s_end; ++step, idx += dir) {
         const double xin = grid[idx], xout = grid[idx + dir];
         const double dxd = xout - xin;
@@ -361,6 +374,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
     }
 #pragma unroll
     for (int h = 0; h < H; ++h) Yg[t + T * h] = ys[0][t + T * h];
+    if (PRED && a.fprev && have_prev) {
+        cx<R> *fp = a.fprev + (size_t)ch * 2 * M;
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            fp[t + T * h] = fs[0][t + T * h];
+            fp[M + t + T * h] = fs[1][t + T * h];
+        }
+    }
     if (t == 0) {
         a.its[ch] += its_sum;
         if (st) a.status[ch] = st;

@@ -73,6 +73,15 @@ def test_grid_matches_oracle():
     assert np.array_equal(W.grid(s), O.Oracle(cfg).grid())
 
 
+def test_smoke_shape_shoots_in_the_oracle():
+    """The smoke entry point's shape (Ntau=128, 1000 x-points) shoots in the
+    oracle; 600 and 800 x-points fail its IRK iteration."""
+    s = W.config(1)
+    cfg = O.Config(Ntau=s.Ntau, NLeft=s.NLeft, NRight=s.NRight, XMid=s.XMid, Delta=s.Delta)
+    F, st = O.Oracle(cfg).shoot(W.initial_x(s))
+    assert st == 0 and np.all(np.isfinite(F))
+
+
 @pytest.mark.slow
 def test_batched_problems_survive_the_oracle_shoot():
     """Every problem of the batched workload shoots in the FP64 oracle at its
