@@ -25,7 +25,7 @@ SOURCES = [os.path.join(CSRC, "cc_capi.hip")] + \
 HEADERS = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hpp")] + \
           [os.path.join(ROOT, "include", "cc_gpu.h")]
 
-HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("newton.cpp", "cc_newton.cpp")]
+HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("newton.cpp", "suite.cpp", "cc_newton.cpp")]
 CLI = os.path.join(HERE, "cc_newton")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -114,6 +114,23 @@ __device__ __forceinline__ void dft(cx<R>* v)
 // remaining factor.  All but the last pass exchange through LDS (B*N
 // complex); the last pass's outputs land back in layout L.
 // tw[e] = (cos, sin)(2 pi e / N), e in [0, N).
+// Exchange-buffer position of element i.  Stockham writes of the early
+// passes put consecutive lanes 4 (or 16) elements apart, so 16 lanes would
+// share 4 of the 16 complex bank slots; XOR-ing both low bit pairs with bits
+// 4-5 of i spreads every pass's writes and the layout-L reads over all 16
+// slots.  The map permutes each aligned block of 16, so positions stay in
+// [0, N).
+template <int N>
+__device__ __forceinline__ int xswz(int i)
+{
+    if constexpr (N >= 64) {
+        const int g = (i >> 4) & 3;
+        return i ^ (g | (g << 2));
+    } else {
+        return i;
+    }
+}
+
 // DB = 1: the exchanges alternate between two LDS buffers (2*B*N complex),
 // so each exchange needs one barrier instead of two: a thread writing buffer
 // k+2 has passed exchange k+1's barrier, which every thread reaches only
@@ -173,7 +190,7 @@ __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const
             } else {
                 const int base = (j / NS) * NS * RAD + k;
 #pragma unroll
-                for (int r = 0; r < RAD; ++r) lds[PAR * B * N + s * N + base + r * NS] = u[r];
+                for (int r = 0; r < RAD; ++r) lds[PAR * B * N + s * N + xswz<N>(base + r * NS)] = u[r];
             }
         }
     }
@@ -182,7 +199,7 @@ __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const
 #pragma unroll
         for (int s = 0; s < B; ++s)
 #pragma unroll
-            for (int p = 0; p < P; ++p) v[s][p] = lds[PAR * B * N + s * N + t + T * p];
+            for (int p = 0; p < P; ++p) v[s][p] = lds[PAR * B * N + s * N + xswz<N>(t + T * p)];
         if constexpr (!DB) __syncthreads();
         fft_pass<N, T, SIGN, B, NS * RAD, R, DB, DB ? 1 - PAR : 0>(v, lds, tw, t);
     }

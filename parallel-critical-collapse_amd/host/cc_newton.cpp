@@ -1,16 +1,22 @@
-// cc_newton -- command-line driver of the GPU Newton solver, the single-run
-// and benchmark modes of the reference's main (src/main.cpp:120-234) on one
-// simulation JSON.
+// cc_newton -- command-line driver of the GPU Newton solver: the single-run,
+// multiple-run and benchmark modes of the reference's main (src/main.cpp:19-378).
 //
-//   cc_newton [-i input.json] [-o output.json] [--precision mixed|fp64]
-//             [--autotune] [--graph] [--reuse-sides] [--ignore-converged]
-//             [--benchmark] [--benchmark-repetitions K] [--quiet] [--dry-run]
+//   cc_newton [-s | -m] [-r] [-i input.json] [-o output.json]
+//             [--precision mixed|fp64] [--autotune] [--graph] [--reuse-sides]
+//             [--ignore-converged] [--benchmark] [--benchmark-repetitions K]
+//             [--quiet] [--dry-run]
 //
-// Single run: the result dictionary (NewtonSolver.cpp:775-816) is written to
-// output.json, by default back into input.json as the reference does
-// (main.cpp:223-233).  Benchmark: K repetitions, per-repetition NewtonStep_<i>
-// and OverallTime, written to benchmark_D=<Dim>_GPU_<policy>.json next to the
-// input (main.cpp:136-201).  --dry-run parses the input and prints it back.
+// Single run (-s, default): the result dictionary (NewtonSolver.cpp:775-816)
+// is written to output.json, by default back into input.json as the reference
+// does (main.cpp:223-233).  Multiple run (-m, -r for ascending order): the
+// input is a suite keyed by dimension; each dimension is solved in the suite's
+// order with initial data from the file, copied or extrapolated in D, and the
+// whole suite is rewritten after each one (main.cpp:236-378).  Benchmark: K
+// repetitions of one simulation, per-repetition NewtonStep_<i> and
+// OverallTime, written to benchmark_D=<Dim>_GPU_<policy>.json next to the
+// input (main.cpp:136-201).  --dry-run parses the input and prints it back; with
+// -m it prints the run order and where each dimension's initial data would come
+// from, and -o receives the suite with the initial data filled in.
 #include <cstdio>
 #include <filesystem>
 #include <fstream>
@@ -18,24 +24,70 @@
 #include <string>
 
 #include "newton.hpp"
+#include "suite.hpp"
 
 using cchost::Json;
 
 namespace {
 
-void write_json(const std::string &path, const Json &j)
+void write_json(const std::string &path, const Json &j, int indent = 4)
 {
     std::ofstream f(path);
     if (!f) throw std::runtime_error("cannot write " + path);
-    f << j.dump(4) << "\n";
+    f << j.dump(indent) << "\n";
 }
 
 int usage()
 {
-    std::cerr << "usage: cc_newton [-i input.json] [-o output.json] [--precision mixed|fp64] [--autotune]\n"
-                 "                 [--graph] [--reuse-sides] [--ignore-converged] [--benchmark]\n"
+    std::cerr << "usage: cc_newton [-s | -m] [-r] [-i input.json] [-o output.json] [--precision mixed|fp64]\n"
+                 "                 [--autotune] [--graph] [--reuse-sides] [--ignore-converged] [--benchmark]\n"
                  "                 [--benchmark-repetitions K] [--quiet] [--dry-run]\n";
     return EXIT_FAILURE;
+}
+
+const char *source_name(cchost::InitialData d)
+{
+    switch (d) {
+    case cchost::InitialData::File: return "from the input";
+    case cchost::InitialData::Copied: return "copied from the previous dimension";
+    default: return "extrapolated (quadratic in D) from the previous three dimensions";
+    }
+}
+
+// main.cpp:236-357
+int multiple_run(const std::string &input, const std::string &output, bool reversed, bool ignore_converged,
+                 bool dry, const cchost::SolverOptions &opt)
+{
+    cchost::SimulationSuite suite = cchost::SimulationSuite::load(input, reversed, ignore_converged);
+    const std::string path = output.empty() ? input : output;
+    std::cout << "Starting multi run for " << suite.dims.size() << " distinct dimensions.\n\n";
+    Json planned = Json::object();
+    for (size_t i = 0; i < suite.dims.size(); ++i) {
+        const std::string &dim = suite.dims[i];
+        cchost::SimulationConfig config = suite.generate(dim);
+        std::cout << "Simulation " << i + 1 << "/" << suite.dims.size() << ":\n";
+        std::cout << "Starting simulation for D=" << dim << ".\n\n";
+        if (dry) {
+            try {
+                const cchost::InitialData src = cchost::prepare_initial_data(suite, i, config);
+                std::cout << "initial data " << source_name(src) << "\n\n";
+                planned[dim] = config.to_json();
+            } catch (const std::runtime_error &e) {
+                std::cout << "no initial data yet: " << e.what() << "\n\n";
+                planned[dim] = suite.dict[dim];
+            }
+            continue;
+        }
+        const cchost::InitialData src = cchost::prepare_initial_data(suite, i, config);
+        if (opt.verbose) std::cout << "initial data " << source_name(src) << "\n\n";
+        if (ignore_converged) config.Converged = false;
+        cchost::NewtonSolver solver(config, opt, false);
+        suite.dict[dim] = solver.run();
+        std::cout << "Result stored in file: " << path << "\n\n";
+        write_json(path, suite.dict, 0);
+    }
+    if (dry && !output.empty()) write_json(output, planned, 4);
+    return 0;
 }
 
 }  // namespace
@@ -43,7 +95,7 @@ int usage()
 int main(int argc, char *argv[])
 {
     std::string input = "data/simulation_4D_512.json", output;
-    bool ignore_converged = false, benchmark = false, dry = false;
+    bool ignore_converged = false, benchmark = false, dry = false, single = true, reversed = false;
     int reps = 3;
     cchost::SolverOptions opt;
     try {
@@ -70,13 +122,16 @@ int main(int argc, char *argv[])
                 reps = std::stoi(next());
             } else if (a == "--quiet") opt.verbose = false;
             else if (a == "--dry-run") dry = true;
-            else if (a == "-s" || a == "--single-run") {
-            } else return usage();
+            else if (a == "-s" || a == "--single-run") single = true;
+            else if (a == "-m" || a == "--multiple-run") single = false;
+            else if (a == "-r" || a == "--reversed-order") reversed = true;
+            else return usage();
         }
         if (!std::filesystem::exists(input)) throw std::invalid_argument("Invalid simulation input path!");
+        opt.debug_dir = std::filesystem::absolute(input).parent_path().string();
+        if (!single && !benchmark) return multiple_run(input, output, reversed, ignore_converged, dry, opt);
         cchost::SimulationConfig config = cchost::SimulationConfig::load(input);
         if (ignore_converged) config.Converged = false;
-        opt.debug_dir = std::filesystem::absolute(input).parent_path().string();
         if (dry) {
             std::cout << config.to_json().dump(4) << std::endl;
             return 0;

@@ -112,3 +112,35 @@ def test_cc_newton_debug_writes_jacobians(anchored_config, tmp_path):
     J_ref = ctx.last_jacobian()
     assert J.shape == J_ref.shape
     assert np.max(np.abs(J - J_ref)) <= 1e-15 * np.max(np.abs(J_ref)) * 10   # 16 significant digits in the file
+
+
+def test_cc_newton_multiple_run(anchored_config, tmp_path):
+    """-m -r over D = 4.00 (anchored data), 4.01, 4.02 (each starting from the
+    previous converged solution) and 4.03 (from the quadratic extrapolation of
+    the three): every dimension converges and the suite file holds each
+    result under its key (main.cpp:236-378)."""
+    shape, _, d, _ = anchored_config
+    suite = {"4.00": dict(d, MaxIterNewton=12)}
+    for D in ("4.01", "4.02", "4.03"):
+        m = dict(d, Dim=float(D), MaxIterNewton=12)
+        m["Initial_Conditions"] = {}
+        suite[D] = m
+    p = tmp_path / "suite.json"
+    p.write_text(json.dumps(suite))
+    print("cc_newton -m -r, 4 dimensions at Ntau=128 X=1000", flush=True)
+    r = subprocess.run([CLI, "-m", "-r", "-i", str(p), "--precision", "mixed"], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    assert "extrapolated (quadratic in D)" in r.stdout and r.stdout.count("copied from the previous") == 2
+    res = json.loads(p.read_text())
+    assert list(res) == ["4.00", "4.01", "4.02", "4.03"]
+    deltas = []
+    for D, v in res.items():
+        assert v["Converged"] is True and v["Dim"] == float(D)
+        assert v["mismatchNorm"] < d["PrecisionNewton"]
+        deltas.append(v["Initial_Conditions"]["Delta"])
+    # the converged Delta moves smoothly with D: the quadratic through the
+    # first three predicts the fourth to far better than the step between them
+    c = np.polyfit([4.00, 4.01, 4.02], deltas[:3], 2)
+    assert abs(np.polyval(c, 4.03) - deltas[3]) < 0.1 * abs(deltas[3] - deltas[2])
