@@ -113,6 +113,19 @@ cc_status cc_synchronize(cc_ctx *ctx);
  * jacobian_<Dim>_<i>.csv in Debug runs (NewtonSolver.cpp:337-341). */
 cc_status cc_last_jacobian(cc_ctx *ctx, double *J);
 
+/* Field snapshots of the base shoot at x, the Debug output of
+ * ShootingSolver::integrateToMatchPoint (ShootingSolver.cpp:85-97) that
+ * NewtonSolver writes as fields_<Dim>_<its>.json (NewtonSolver.cpp:287-295):
+ * the left side at grid indices i in [0, NLeft) with i % every == 0 or
+ * i == NLeft-1 (ascending), then the right side at i in (NLeft, NLeft+NRight]
+ * with i % every == 0, i == NLeft+1 or i == NLeft+NRight (descending), each
+ * the state reached at x_i, integrated from the reference's start.  Snapshot
+ * k: xs[k] = x_i, fields[k][4][Ntau] = A = sqrt(1/IA2), U, V, F.  *count
+ * receives the number of snapshots; capacity below it gives CC_BADARG with
+ * nothing written. */
+cc_status cc_shoot_fields(cc_ctx *ctx, const double *x, int32_t every, int32_t capacity, int32_t *count,
+                          double *xs, double *fields);
+
 /* Counters of the last Jacobian/Newton launch: [0] IRK steps, [1] fixed-point
  * iterations (sum of itsReached), [2] chains integrated. */
 cc_status cc_counters(cc_ctx *ctx, int64_t out[3]);

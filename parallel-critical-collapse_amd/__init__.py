@@ -118,6 +118,7 @@ def load():
     L.cc_expansions.argtypes = [_vp, _dp, _dp, _dp]
     L.cc_profile.argtypes = [_vp, C.c_int32]
     L.cc_chain_timing.argtypes = [_vp, _dp, _ip, _lp, _lp]
+    L.cc_shoot_fields.argtypes = [_vp, _dp, C.c_int32, C.c_int32, _ip, _dp, _dp]
     L.cc_pack.argtypes = [_vp, _dp, _dp, C.c_int32]
     L.cc_unpack.argtypes = [_vp, _dp, _dp, C.c_int32]
     _pp = C.POINTER(_dp)
@@ -191,6 +192,20 @@ class Context:
         J = np.zeros((self.n, self.n))
         self._check(load().cc_last_jacobian(self.h, _d(J)), "cc_last_jacobian")
         return J
+
+    def shoot_fields(self, x, every):
+        """Debug field snapshots of the base shoot (cc_shoot_fields): grid
+        abscissae xs [k] and fields [k][4][Ntau] = A, U, V, F."""
+        x = _arr(x)
+        cnt = C.c_int32(0)
+        st = load().cc_shoot_fields(self.h, _d(x), int(every), 0, C.byref(cnt), None, None)
+        if st != CC_BADARG or cnt.value < 1:
+            self._check(st, "cc_shoot_fields")
+        xs = np.zeros(cnt.value)
+        fields = np.zeros((cnt.value, 4, self.N))
+        self._check(load().cc_shoot_fields(self.h, _d(x), int(every), cnt.value, C.byref(cnt), _d(xs), _d(fields)),
+                    "cc_shoot_fields")
+        return xs, fields
 
     def residual(self, x):
         x = _arr(x)

@@ -1673,6 +1673,124 @@ cc_status cc_rhs_eval(cc_ctx *c, const double *Y, const double *xs, double *dY, 
     return CC_OK;
 }
 
+cc_status cc_shoot_fields(cc_ctx *c, const double *x, int32_t every, int32_t capacity, int32_t *count, double *xs,
+                          double *fields)
+{
+    if (!c || !x || !count || every < 1 || capacity < 0) return CC_BADARG;
+    const int N = c->N, M = c->M, n = c->n, iM = c->iM, iR = c->iR;
+    std::vector<int> li, ri;        // snapshot grid indices of each side, in the reference's order
+    for (int i = 0; i < iM; ++i)
+        if (i % every == 0 || i == iM - 1) li.push_back(i);
+    for (int i = iR; i > iM; --i)
+        if (i % every == 0 || i == iM + 1 || i == iR) ri.push_back(i);
+    const int total = (int)(li.size() + ri.size());
+    *count = total;
+    if (capacity < total || !xs || !fields) {
+        c->err = "cc_shoot_fields: " + std::to_string(total) + " snapshots need that capacity";
+        return CC_BADARG;
+    }
+    std::vector<double> g((size_t)c->X + 1);
+    HIPCHK(c, hipMemcpy(g.data(), c->grid, sizeof(double) * g.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(c->x, x, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
+    const int npairs = (total + 1) / 2;
+    const size_t step = (size_t)rec_step_values(N), stage = step / 2;
+    std::vector<double> sxh(2 * (size_t)npairs), sdh(2 * (size_t)npairs, x[N / 2 + 2]);
+    cx<double> *snap = nullptr, *dY = nullptr;
+    double *sx = nullptr, *sd = nullptr, *rec = nullptr;
+    hipError_t e = hipMalloc(&snap, sizeof(cx<double>) * 2 * npairs * M);
+    if (e == hipSuccess) e = hipMalloc(&dY, sizeof(cx<double>) * 2 * npairs * M);
+    if (e == hipSuccess) e = hipMalloc(&sx, sizeof(double) * 2 * npairs);
+    if (e == hipSuccess) e = hipMalloc(&sd, sizeof(double) * 2 * npairs);
+    if (e == hipSuccess) e = hipMalloc(&rec, sizeof(double) * step * npairs);
+    auto release = [&]() {
+        for (void *q : {(void *)snap, (void *)dY, (void *)sx, (void *)sd, (void *)rec}) (void)hipFree(q);
+    };
+    if (e != hipSuccess) {
+        release();
+        HIPCHK(c, e);
+    }
+    const Counts k{1, 1, 2, 1};                 // the base column: chain 0 left, chain 1 right
+    cc_status st = CC_OK;
+    dispatch_n(N, [&](auto Nc) {
+        using O = Ops<decltype(Nc)::value>;
+        st = launch_prologue<decltype(Nc)::value>(c, c->base, k, c->x, 0.0, c->s0);
+        int slot = 0;
+        for (int side = 0; side < 2 && st == CC_OK && e == hipSuccess; ++side) {
+            ccd::ChainParams<double> a = chain_params<decltype(Nc)::value>(c, c->base);
+            a.Y += (size_t)side * M;
+            a.start += side;
+            a.nsteps += side;
+            a.dir += side;
+            a.delta += side;
+            a.status += side;
+            a.fail_idx += side;
+            a.its += side;
+            int done = 0;
+            for (int i : side ? ri : li) {
+                const int steps = side ? iR - i : i;
+                if (steps > done && e == hipSuccess) {
+                    a.step_begin = done;
+                    a.step_end = steps;
+                    e = O::chain(c->s0, O::kThreads, 1, a);
+                    done = steps;
+                }
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(snap + (size_t)slot * M, a.Y, sizeof(cx<double>) * M, hipMemcpyDeviceToDevice,
+                                       c->s0);
+                sxh[slot++] = g[i];
+            }
+        }
+        if (st == CC_OK && e == hipSuccess && (total & 1)) {   // pad the last pair with a copy
+            e = hipMemcpyAsync(snap + (size_t)total * M, snap + (size_t)(total - 1) * M, sizeof(cx<double>) * M,
+                               hipMemcpyDeviceToDevice, c->s0);
+            sxh[total] = sxh[total - 1];
+        }
+        if (st == CC_OK && e == hipSuccess)
+            e = hipMemcpyAsync(sx, sxh.data(), sizeof(double) * sxh.size(), hipMemcpyHostToDevice, c->s0);
+        if (st == CC_OK && e == hipSuccess)
+            e = hipMemcpyAsync(sd, sdh.data(), sizeof(double) * sdh.size(), hipMemcpyHostToDevice, c->s0);
+        if (st == CC_OK && e == hipSuccess)
+            e = O::rhs_pairs(c->s0, false, npairs, snap, sx, sd, dY, c->tw64, c->invm64, c->p.Dim, rec);
+    });
+    std::vector<double> rh(step * npairs);
+    int32_t chst[2] = {0, 0};
+    if (st == CC_OK && e == hipSuccess)
+        e = hipMemcpyAsync(rh.data(), rec, sizeof(double) * rh.size(), hipMemcpyDeviceToHost, c->s0);
+    if (st == CC_OK && e == hipSuccess)
+        e = hipMemcpyAsync(chst, c->base.status, sizeof(chst), hipMemcpyDeviceToHost, c->s0);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s0);
+    release();
+    CCCHK(st);
+    HIPCHK(c, e);
+    if (chst[0] || chst[1]) {
+        c->err = "IRK fixed-point iteration did not converge in the field-snapshot shoot";
+        return CC_IRK_NOCONV;
+    }
+    int oU = 0, oV = 0, oF = 0, oR = 0;          // record offsets of U, V, F (j < M) and r = 1/IA2
+    dispatch_n(N, [&](auto Nc) {
+        using RC = ccd::Rec<decltype(Nc)::value>;
+        oU = RC::U;
+        oV = RC::V;
+        oF = RC::F;
+        oR = RC::R;
+    });
+    for (int s = 0; s < total; ++s) {
+        const double *r = rh.data() + (size_t)(s / 2) * step + (size_t)(s % 2) * stage;
+        double *A = fields + (size_t)s * 4 * N, *U = A + N, *V = U + N, *F = V + N;
+        for (int j = 0; j < M; ++j) {       // second half by the symmetries of ICG.cpp:819-824
+            U[j] = r[oU + j];
+            V[j] = r[oV + j];
+            F[j] = r[oF + j];
+            U[j + M] = -U[j];
+            V[j + M] = -V[j];
+            F[j + M] = F[j];
+        }
+        for (int j = 0; j < N; ++j) A[j] = std::sqrt(r[oR + j]);    // ShootingSolver.cpp:91
+        xs[s] = sxh[s];
+    }
+    return CC_OK;
+}
+
 cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, const int32_t *start,
                        const int32_t *end, int32_t batch, int32_t fp32, int64_t *its_total, int32_t *status)
 {

@@ -299,8 +299,11 @@ constexpr int chain_waves() { return sizeof(R) == 8 ? (N / T >= 8 ? 1 : 2) : 3; 
 // RT: element type of the base-trajectory record (float for the FP32 tangent,
 // double for the FP64 one); unused when a.rec is null.  DB = 1 double-buffers
 // the transform exchanges (one barrier each instead of two) at twice the LDS:
-// for the few latency-bound base chains of a tangent policy, not for the
-// chip-filling FP64 Jacobian launches that need two workgroups per CU.
+// for the few latency-bound base chains, not for the chip-filling FP64
+// Jacobian launches that need two workgroups per CU.  Every step starts the
+// fixed point from the reference's K_s = y (ODEStepper.cpp:193-199): F(x)
+// depends on the start at the level of the iteration tolerance, which the
+// backward integration from the horizon amplifies to ~1e-12.
 template <int N, int T, typename R, typename RT = float, int DB = 0>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N, T, R>()))) void k_chain(ChainParams<R> a)
 {
@@ -401,10 +404,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
 }
 
 // computeDerivatives on pairs of states (unit entry point).  Y, dY: [2*npairs][M];
-// xs, delta: [2*npairs]; both members of a pair share Delta.
+// xs, delta: [2*npairs]; both members of a pair share Delta.  rec (may be
+// null): per pair one Rec<N> step, the fields of member s in stage slot s
+// (StateVectorToFields with derivatives, ICG.cpp:791-884).
 template <int N, int T, typename R>
 __global__ __launch_bounds__(T) void k_rhs_pairs(const cx<R> *Y, const double *xs, const double *delta,
-                                                 cx<R> *dY, const cx<R> *tw, const R *invm, double Dim)
+                                                 cx<R> *dY, const cx<R> *tw, const R *invm, double Dim, R *rec)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
     __shared__ cx<R> xb[2 * N];
@@ -423,7 +428,7 @@ __global__ __launch_bounds__(T) void k_rhs_pairs(const cx<R> *Y, const double *x
     __syncthreads();
     const RhsConst<R> K = make_const<N, R>(Dim, delta[2 * pr]);
     const R xsr[2] = {(R)xs[2 * pr], (R)xs[2 * pr + 1]};
-    rhs_pair<N, T, R>(yk, f, xsr, K, invm, xb, bc, tw, t);
+    rhs_pair<N, T, R, R>(yk, f, xsr, K, invm, xb, bc, tw, t, rec ? rec + (size_t)pr * Rec<N>::STEP : nullptr);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll

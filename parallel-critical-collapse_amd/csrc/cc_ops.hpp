@@ -34,7 +34,7 @@ struct Ops {
                                double *delta);
     // k_chain; FP32 chains run at kThreads only.  a.rec (if set) is written as
     // double when rec_fp64 (kThreads only), else as float.  Launches of at most
-    // kFewChains chains at kThreads double-buffer the transform exchanges.
+    // kFewChains chains run the double-buffered variant.
     static hipError_t chain(hipStream_t s, int T, int nchains, const ChainParams<double> &a, bool rec_fp64 = false);
     static hipError_t chain(hipStream_t s, int T, int nchains, const ChainParams<float> &a);
     static hipError_t mismatch(hipStream_t s, bool fp32, int ncols, const void *Y, const int32_t *lidx,
@@ -55,7 +55,8 @@ struct Ops {
     static hipError_t fft(hipStream_t s, bool fp32, int sign, int batch, const cx<double> *in, cx<double> *out,
                           const void *tw);
     static hipError_t rhs_pairs(hipStream_t s, bool fp32, int npairs, const void *Y, const double *xs,
-                                const double *delta, void *dY, const void *tw, const void *invm, double Dim);
+                                const double *delta, void *dY, const void *tw, const void *invm, double Dim,
+                                void *rec = nullptr);
 };
 
 extern template struct Ops<128>;

@@ -35,26 +35,27 @@ hipError_t Ops<N>::boundary(hipStream_t s, bool left, bool fp32, int ncols, cons
 template <int N>
 hipError_t Ops<N>::chain(hipStream_t s, int T, int nchains, const ChainParams<double> &a, bool rec_fp64)
 {
-    // Up to one workgroup per CU, the doubled LDS of the double-buffered
-    // exchanges costs no occupancy: the base chains of the tangent policies and
-    // the residual.  The exchanges move the same values, so both variants give
-    // bit-identical chains.
+    // Launches of up to one workgroup per CU (base chains, the residual) run
+    // the double-buffered variant: its doubled LDS costs no occupancy there.
+    // The exchanges move the same values, so both variants give identical chains.
     constexpr int DB = N <= 1024 ? 1 : 0;
-    const bool few = nchains <= kFewChains;
+    const bool dbl = nchains <= kFewChains;
+    const dim3 g((unsigned)nchains);
     if (rec_fp64) {
         if (T != kThreads) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_chain<N, kThreads, double, double, DB>), dim3((unsigned)nchains), dim3(kThreads), 0, s,
-                           a);
-    } else if (few && T == kThreads) {
-        hipLaunchKernelGGL((k_chain<N, kThreads, double, float, DB>), dim3((unsigned)nchains), dim3(kThreads), 0, s,
-                           a);
+        hipLaunchKernelGGL((k_chain<N, kThreads, double, double, DB>), g, dim3(kThreads), 0, s, a);
     } else if (T == kThreads) {
-        hipLaunchKernelGGL((k_chain<N, kThreads, double>), dim3((unsigned)nchains), dim3(kThreads), 0, s, a);
+        if (dbl)
+            hipLaunchKernelGGL((k_chain<N, kThreads, double, float, DB>), g, dim3(kThreads), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_chain<N, kThreads, double>), g, dim3(kThreads), 0, s, a);
     } else {
         if constexpr (kAltThreads != 0) {
             if (T != kAltThreads) return hipErrorInvalidValue;
-            hipLaunchKernelGGL((k_chain<N, kAltThreads, double>), dim3((unsigned)nchains), dim3(kAltThreads), 0, s,
-                               a);
+            if (dbl)
+                hipLaunchKernelGGL((k_chain<N, kAltThreads, double, float, DB>), g, dim3(kAltThreads), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_chain<N, kAltThreads, double>), g, dim3(kAltThreads), 0, s, a);
         } else {
             return hipErrorInvalidValue;
         }
@@ -183,17 +184,20 @@ hipError_t Ops<N>::fft(hipStream_t s, bool fp32, int sign, int batch, const cx<d
 
 template <int N>
 hipError_t Ops<N>::rhs_pairs(hipStream_t s, bool fp32, int npairs, const void *Y, const double *xs,
-                             const double *delta, void *dY, const void *tw, const void *invm, double Dim)
+                             const double *delta, void *dY, const void *tw, const void *invm, double Dim,
+                             void *rec)
 {
     constexpr int T = kThreads;
     if (fp32)
         hipLaunchKernelGGL((k_rhs_pairs<N, T, float>), dim3((unsigned)npairs), dim3(T), 0, s,
                            static_cast<const cx<float> *>(Y), xs, delta, static_cast<cx<float> *>(dY),
-                           static_cast<const cx<float> *>(tw), static_cast<const float *>(invm), Dim);
+                           static_cast<const cx<float> *>(tw), static_cast<const float *>(invm), Dim,
+                           static_cast<float *>(rec));
     else
         hipLaunchKernelGGL((k_rhs_pairs<N, T, double>), dim3((unsigned)npairs), dim3(T), 0, s,
                            static_cast<const cx<double> *>(Y), xs, delta, static_cast<cx<double> *>(dY),
-                           static_cast<const cx<double> *>(tw), static_cast<const double *>(invm), Dim);
+                           static_cast<const cx<double> *>(tw), static_cast<const double *>(invm), Dim,
+                           static_cast<double *>(rec));
     return hipGetLastError();
 }
 

@@ -293,11 +293,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
                     A11 = (R)(dxd * a.a11);
         const R B0 = (R)(dxd * a.b0), B1 = (R)(dxd * a.b1);
         const R *rec = a.rec + (size_t)(a.rec_base[ch] + step) * Rec<N>::STEP;
-        // Initial stage iterates: the step-start state, or (after the first
-        // step of a launch) the stages the previous step's derivatives predict,
+        // Initial stage iterates: the step-start state, or (after a chain's
+        // first step) the stages the previous step's derivatives predict,
         // K_s = y + dx sum_j a_sj f_j(prev).  The linear fixed point then starts
-        // O(dx^2) instead of O(dx) from its solution (the base chains keep the
-        // reference's start, ODEStepper.cpp:193-199).
+        // O(dx^2) instead of O(dx) from its solution (the reference starts
+        // from K_s = y, ODEStepper.cpp:193-199).
 #pragma unroll
         for (int h = 0; h < H; ++h) {
             const cx<R> y = ys[0][t + T * h];

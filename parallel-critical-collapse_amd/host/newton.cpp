@@ -1,6 +1,7 @@
 // newton.cpp -- SimulationConfig and NewtonSolver over the C-ABI (see newton.hpp).
 #include "newton.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <fstream>
@@ -186,6 +187,43 @@ void NewtonSolver::write_jacobian_csv(size_t iteration) const
     }
 }
 
+// common.cpp:201-215: elements i % n == 0 and the last one
+static std::vector<double> every_nth_element(const double *a, size_t size, size_t n)
+{
+    std::vector<double> out;
+    for (size_t i = 0; i < size; ++i)
+        if (i % n == 0 || i + 1 == size) out.push_back(a[i]);
+    return out;
+}
+
+// NewtonSolver.cpp:287-295 with ShootingSolver.cpp:85-97: Debug field
+// snapshots of the shoot at x, keyed by std::to_string(x_i), written as
+// fields_<Dim>_<iteration>.json (compact, as OutputWriter::writeJsonToFile).
+void NewtonSolver::write_fields_json(size_t iteration) const
+{
+    int32_t count = 0;
+    const cc_status st0 = cc_shoot_fields(ctx_, x_.data(), cfg_.DebugNx, 0, &count, nullptr, nullptr);
+    if (st0 != CC_BADARG || count < 1) check(st0, "cc_shoot_fields");
+    std::vector<double> xs(count), f((size_t)count * 4 * N_);
+    check(cc_shoot_fields(ctx_, x_.data(), cfg_.DebugNx, count, &count, xs.data(), f.data()), "cc_shoot_fields");
+    Json out = Json::object();
+    const size_t every = (size_t)std::max(1, cfg_.DebugNtau);
+    for (int k = 0; k < count; ++k) {
+        const double *A = f.data() + (size_t)k * 4 * N_;
+        Json e = Json::object();
+        e["A"] = Json(every_nth_element(A, N_, every));
+        e["U"] = Json(every_nth_element(A + N_, N_, every));
+        e["V"] = Json(every_nth_element(A + 2 * N_, N_, every));
+        e["f"] = Json(every_nth_element(A + 3 * N_, N_, every));
+        out[std::to_string(xs[k])] = e;
+    }
+    const std::string path =
+        opt_.debug_dir + "/fields_" + std::to_string(cfg_.Dim) + "_" + std::to_string(iteration) + ".json";
+    std::ofstream fo(path);
+    if (!fo) throw std::runtime_error("Could not open file for writing: " + path);
+    fo << out.dump(0) << "\n";
+}
+
 // NewtonSolver.cpp:763-768
 double NewtonSolver::l2norm(const std::vector<double> &v)
 {
@@ -267,6 +305,7 @@ Json NewtonSolver::run(Json *bench)
             const double t_step = now_s();
             say("Newton iteration: " + std::to_string(its + 1));
             errOld = err;
+            if (cfg_.Debug) write_fields_json(its);
             check(cc_newton_step(ctx_, x_.data(), F0.data(), dx.data()), "Newton step");
             if (cfg_.Debug) write_jacobian_csv(its + 1);
             err = l2norm(F0);

@@ -89,6 +89,7 @@ private:
     void unpack(const std::vector<double> &x);
     void write_final_output(size_t newton_its, double mismatch);
     void write_jacobian_csv(size_t iteration) const;
+    void write_fields_json(size_t iteration) const;
     static double l2norm(const std::vector<double> &v);
 };
 

@@ -112,6 +112,17 @@ def test_cc_newton_debug_writes_jacobians(anchored_config, tmp_path):
     J_ref = ctx.last_jacobian()
     assert J.shape == J_ref.shape
     assert np.max(np.abs(J - J_ref)) <= 1e-15 * np.max(np.abs(J_ref)) * 10   # 16 significant digits in the file
+    # field snapshots of the shoot of iteration 0 (NewtonSolver.cpp:287-295), keyed by std::to_string(x)
+    fpath = tmp_path / "fields_4.000000_0.json"
+    assert fpath.exists()
+    fj = json.loads(fpath.read_text())
+    xs, fields = ctx.shoot_fields(x0, d2.get("DebugNx", 100))
+    assert list(fj) == [f"{v:.6f}" for v in xs] or len(fj) == len({f"{v:.6f}" for v in xs})
+    k = f"{xs[1]:.6f}"
+    step = d2.get("DebugNtau", 1)
+    keep = [j for j in range(shape.Ntau) if j % step == 0 or j == shape.Ntau - 1]
+    for name, row in zip(("A", "U", "V", "f"), fields[1]):
+        assert fj[k][name] == list(row[keep]), name
 
 
 def test_cc_newton_multiple_run(anchored_config, tmp_path):

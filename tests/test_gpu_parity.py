@@ -309,6 +309,39 @@ def test_mixed_newton_step_solves_fp64_refined_system(c1_reference):
     assert np.linalg.norm(J @ dx + F0) / np.linalg.norm(F0) < 1e-8
 
 
+def test_shoot_fields_match_oracle():
+    """Debug field snapshots (ShootingSolver.cpp:85-97): the states of both
+    sides at the snapshot indices, integrated from the reference's start, and
+    their fields A = sqrt(1/IA2), U, V, F against the oracle's integration and
+    StateVectorToFields with IA2 (ICG.cpp:791-884)."""
+    s = shape_c1(128)
+    sh = s["shape"]
+    every = 100
+    ctx = ctx_for(s, precision=cc.CC_MIXED)
+    xs, fields = ctx.shoot_fields(s["x"], every)
+    o = oracle_for(s)
+    grid = o.grid()
+    _, st, (YL, YR, _, _) = o.shoot_states(s["x"])
+    assert st == 0
+    iM, iR = sh.NLeft, sh.NLeft + sh.NRight
+    li = [i for i in range(iM) if i % every == 0 or i == iM - 1]
+    ri = [i for i in range(iR, iM, -1) if i % every == 0 or i == iM + 1 or i == iR]
+    assert np.array_equal(xs, grid[li + ri])
+    k = 0
+    for Y, idx in ((YL, li), (YR, ri)):
+        at = idx[0]
+        for i in idx:
+            if i != at:
+                Y, st, _ = o.integrate(Y, grid, at, i)
+                assert st == 0
+                at = i
+            U, V, F, IA2, *_ = o.state_to_fields_d(Y, grid[i])
+            for got, ref in zip(fields[k], (np.sqrt(1 / IA2), U, V, F)):
+                assert np.max(np.abs(got - ref)) < 1e-10 * max(1.0, np.max(np.abs(ref))), (k, i)
+            k += 1
+    assert k == len(xs)
+
+
 @pytest.fixture(scope="module")
 def c1_tangent_jacobians(c1_reference):
     s, F0, _, _, _ = c1_reference
