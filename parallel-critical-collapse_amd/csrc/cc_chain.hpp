@@ -100,6 +100,16 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     // Everything derived from the thread index is recomputed per call: held
     // across the fixed-point loop it would cost more registers than it saves.
     asm volatile("" : "+v"(t));
+    // DB: the transforms of one evaluation follow each other without trailing
+    // barriers, each starting in the region its predecessor did not read last
+    // (fft_next_par); the spectrum separation takes the next region and the
+    // transform after it the other one.  The caller's reduction barrier
+    // separates the last transform from the next evaluation.
+    constexpr int RS = 2 * N;
+    constexpr int pA = 0, pB = DB ? fft_next_par<N, T>(pA) : 0, pC = DB ? fft_next_par<N, T>(pB) : 0;
+    constexpr int pD = DB ? fft_next_par<N, T>(pC) : 0, pE = DB ? fft_next_par<N, T>(pD) : 0;
+    constexpr int pS = DB ? fft_next_par<N, T>(pE) : 0, pG = DB ? 1 - pS : 0;
+    constexpr int pH = DB ? fft_next_par<N, T>(pG) : 0;
 
     // per-stage products of the split that survive until the pointwise RHS
     R AU[2][H], BU[2][H], AV[2][H], BV[2][H], CF[2][H];
@@ -136,7 +146,8 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
             z[1][p] = {w * v.y, -(w * v.x)};
         }
         // (2) back to tau (804-805)
-        fft_block<N, T, -1, 2, R, DB>(z, xb, tw, t);
+        if (s == 0) fft_block<N, T, -1, 2, R, DB, pA, 0, RS>(z, xb, tw, t);
+        else fft_block<N, T, -1, 2, R, DB, pB, 0, RS>(z, xb, tw, t);
         // (3) half-period split (814-827), IA2 coefficients (834-836), RHS products
         const R X = xs[s];
         const R inv_x = (R)1 / X;
@@ -172,7 +183,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     for (int h = 0; h < H; ++h) c[h + H] = c[h];
 
     // (4) solveInhom step 1 for both stages: F' = Coeff1 (264-270)
-    fft1<N, T, +1, R, DB>(c, xb, tw, t);
+    fft1<N, T, +1, R, DB, pC, 0, RS>(c, xb, tw, t);
 #pragma unroll
     for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
     if (t == 0) {                       // lambda_s = mean of Coeff1_s
@@ -185,7 +196,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         const cx<R> v = c[p];
         c[p] = {-(v.y * im), v.x * im};
     }
-    fft1<N, T, -1, R, DB>(c, xb, tw, t);
+    fft1<N, T, -1, R, DB, pD, 0, RS>(c, xb, tw, t);
     // (5) h = -g e^{F} (276-279), forward (283); keep e^{F}
     R E[2][P];
 #pragma unroll
@@ -194,17 +205,17 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         E[1][p] = exp(c[p].y);
         c[p] = {K.mg * E[0][p], K.mg * E[1][p]};
     }
-    fft1<N, T, +1, R, DB>(c, xb, tw, t);
+    fft1<N, T, +1, R, DB, pE, 0, RS>(c, xb, tw, t);
 #pragma unroll
     for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
     // (6) separate the two Hermitian spectra, lamIntegrate with lambda_s (284)
 #pragma unroll
-    for (int p = 0; p < P; ++p) xb[t + T * p] = c[p];
+    for (int p = 0; p < P; ++p) xb[pS * RS + t + T * p] = c[p];
     __syncthreads();
     cx<R> cm[P];
 #pragma unroll
-    for (int p = 0; p < P; ++p) cm[p] = xb[(N - (t + T * p)) & (N - 1)];
-    __syncthreads();
+    for (int p = 0; p < P; ++p) cm[p] = xb[pS * RS + ((N - (t + T * p)) & (N - 1))];
+    if constexpr (!DB) __syncthreads();
     const R lam0 = bc[2], lam1 = bc[3];
     if (rec && t == 0) {
         rec[RC::LAM] = (RT)lam0;
@@ -240,7 +251,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         }
         c[p] = {G[0].x - G[1].y, G[0].y + G[1].x};
     }
-    fft1<N, T, -1, R, DB>(c, xb, tw, t);
+    fft1<N, T, -1, R, DB, pG, 0, RS>(c, xb, tw, t);
 
     // (7) pointwise RHS with r = 1/IA2 = e^{F}/X (288-291; ODEStepper.cpp:109-122)
     cx<R> yp[2][P];
@@ -261,7 +272,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         }
     }
     // (8) FieldsToStateVector (766-778) and Re dY[2] = 0 (ODEStepper.cpp:128)
-    fft_block<N, T, +1, 2, R, DB>(yp, xb, tw, t);
+    fft_block<N, T, +1, 2, R, DB, pH, 0, RS>(yp, xb, tw, t);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll

@@ -115,27 +115,33 @@ __device__ __forceinline__ void dft(cx<R>* v)
 // complex); the last pass's outputs land back in layout L.
 // tw[e] = (cos, sin)(2 pi e / N), e in [0, N).
 // Exchange-buffer position of element i.  Stockham writes of the early
-// passes put consecutive lanes 4 (or 16) elements apart, so 16 lanes would
-// share 4 of the 16 complex bank slots; XOR-ing both low bit pairs with bits
-// 4-5 of i spreads every pass's writes and the layout-L reads over all 16
-// slots.  The map permutes each aligned block of 16, so positions stay in
-// [0, N).
-template <int N>
+// passes put consecutive lanes 4 (or 16) elements apart, so the lanes that
+// one LDS cycle serves (16 lanes of 8-byte FP32 complex, 8 lanes of 16-byte
+// FP64 complex: 128 bytes over 32 banks) would share a few bank slots.  FP32:
+// both low bit pairs XOR bits 4-5 of i; FP64: the low three bits XOR a map
+// of bits 3-4.  Either spreads every pass's writes and the layout-L reads
+// over all slots, and permutes each aligned block of 16 (8), so positions
+// stay in [0, N).  The map is linear over GF(2): for a and b with disjoint
+// bits (a + b == a ^ b), xswz(a + b) == xswz(a) ^ xswz(b), so a position
+// made of a per-thread part and a compile-time offset costs one XOR.
+template <int N, typename R>
 __device__ __forceinline__ int xswz(int i)
 {
-    if constexpr (N >= 64) {
+    if constexpr (N < 64) {
+        return i;
+    } else if constexpr (sizeof(R) == 8) {
+        return i ^ (((i >> 3) & 1) ^ (((i >> 4) & 1) * 6));
+    } else {
         const int g = (i >> 4) & 3;
         return i ^ (g | (g << 2));
-    } else {
-        return i;
     }
 }
 
-// DB = 1: the exchanges alternate between two LDS buffers (2*B*N complex),
-// so each exchange needs one barrier instead of two: a thread writing buffer
-// k+2 has passed exchange k+1's barrier, which every thread reaches only
-// after reading buffer k.  PAR is the buffer of this pass.
-template <int N, int T, int SIGN, int B, int NS, typename R, int DB = 0, int PAR = 0>
+// DB = 1: the exchanges alternate between two LDS regions (RS complex apart,
+// RS >= B*N), so each exchange needs one barrier instead of two: a thread
+// writing region k+2 has passed exchange k+1's barrier, which every thread
+// reaches only after reading region k.  PAR is the region of this pass.
+template <int N, int T, int SIGN, int B, int NS, typename R, int DB = 0, int PAR = 0, int RS = B * N>
 __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const cx<R>* __restrict__ tw,
                                          int t)
 {
@@ -188,9 +194,10 @@ __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const
 #pragma unroll
                 for (int r = 0; r < RAD; ++r) v[s][b + r * NB] = u[r];
             } else {
-                const int base = (j / NS) * NS * RAD + k;
+                // base has no bits in common with r * NS (base mod NS*RAD == k < NS)
+                const int sb = xswz<N, R>((j / NS) * NS * RAD + k);
 #pragma unroll
-                for (int r = 0; r < RAD; ++r) lds[PAR * B * N + s * N + xswz<N>(base + r * NS)] = u[r];
+                for (int r = 0; r < RAD; ++r) lds[PAR * RS + s * N + (sb ^ xswz<N, R>(r * NS))] = u[r];
             }
         }
     }
@@ -199,30 +206,57 @@ __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const
 #pragma unroll
         for (int s = 0; s < B; ++s)
 #pragma unroll
-            for (int p = 0; p < P; ++p) v[s][p] = lds[PAR * B * N + s * N + xswz<N>(t + T * p)];
+            for (int p = 0; p < P; ++p) v[s][p] = lds[PAR * RS + s * N + (xswz<N, R>(t) ^ xswz<N, R>(T * p))];
         if constexpr (!DB) __syncthreads();
-        fft_pass<N, T, SIGN, B, NS * RAD, R, DB, DB ? 1 - PAR : 0>(v, lds, tw, t);
+        fft_pass<N, T, SIGN, B, NS * RAD, R, DB, DB ? 1 - PAR : 0, RS>(v, lds, tw, t);
     }
+}
+
+// Number of LDS exchanges of one transform (passes before the last).
+template <int N, int T>
+constexpr int fft_exchanges()
+{
+    int e = 0;
+    for (int ns = 1;;) {
+        const int rad = (N / ns >= N / T) ? N / T : N / ns;
+        if (ns * rad == N) return e;
+        ++e;
+        ns *= rad;
+    }
+}
+
+// Chained double-buffered transforms without trailing barriers.  A transform
+// starting in region p0 reads region p0 last when it has an odd number of
+// exchanges, else region 1-p0; the next LDS writer may start at once in the
+// other region (every thread passed the last exchange's barrier only after
+// reading the region before it), which is what fft_next_par returns.
+template <int N, int T>
+constexpr int fft_next_par(int p0)
+{
+    return (fft_exchanges<N, T>() % 2 == 1) ? 1 - p0 : p0;
 }
 
 // Batched block FFT: B signals of length N in layout L, in place.
 // Requires N/T >= 2 and a __syncthreads-safe caller (all T threads call it).
-// lds holds B*N complex, or 2*B*N with DB = 1 (one trailing barrier then
-// protects the buffers from the caller's next LDS writes).
-template <int N, int T, int SIGN, int B, typename R, int DB = 0>
+// lds holds B*N complex, or with DB = 1 two regions RS apart starting with
+// region P0.  TRAIL = 1 (DB) ends with a barrier that protects both regions
+// from the caller's next LDS writes; TRAIL = 0 leaves that to the caller
+// (see fft_next_par).
+template <int N, int T, int SIGN, int B, typename R, int DB = 0, int P0 = 0, int TRAIL = 1, int RS = B * N>
 __device__ __forceinline__ void fft_block(cx<R> (&v)[B][N / T], cx<R>* lds, const cx<R>* __restrict__ tw, int t)
 {
     static_assert((N & (N - 1)) == 0 && N / T >= 2 && N / T <= 16, "unsupported FFT shape");
-    fft_pass<N, T, SIGN, B, 1, R, DB, 0>(v, lds, tw, t);
-    if constexpr (DB) __syncthreads();
+    static_assert(DB || P0 == 0, "one exchange region without double buffering");
+    fft_pass<N, T, SIGN, B, 1, R, DB, P0, RS>(v, lds, tw, t);
+    if constexpr (DB && TRAIL) __syncthreads();
 }
 
 // Single-signal convenience wrapper.
-template <int N, int T, int SIGN, typename R, int DB = 0>
+template <int N, int T, int SIGN, typename R, int DB = 0, int P0 = 0, int TRAIL = 1, int RS = N>
 __device__ __forceinline__ void fft1(cx<R> (&v)[N / T], cx<R>* lds, const cx<R>* __restrict__ tw, int t)
 {
     cx<R>(&vb)[1][N / T] = *reinterpret_cast<cx<R>(*)[1][N / T]>(&v);
-    fft_block<N, T, SIGN, 1, R, DB>(vb, lds, tw, t);
+    fft_block<N, T, SIGN, 1, R, DB, P0, TRAIL, RS>(vb, lds, tw, t);
 }
 
 // Workgroup sum of one value per thread (T threads, T a multiple of 64).
