@@ -33,9 +33,10 @@ sys.path.insert(0, ROOT)
 # HBM traffic per launch of the dominant kernel, from two rocprofv3 --pmc
 # passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a plain
 # device copy; see profiles/hbm_traffic_newton_step_ntau1024_x8000.md.  Keyed
-# by the kernel the bench reports; a kernel without a measurement reports null.
+# by the kernel the bench reports and the steps per launch it was measured at
+# (the record it reads grows with them); anything else reports null.
 MEASURED_TRAFFIC_BYTES = {
-    "k_tangent<1024,256>": (98.1e6 + 3.2e6),
+    ("k_tangent<1024,256>", 64): (54.27e6 + 13.87e6),
 }
 
 
@@ -341,7 +342,7 @@ def main():
     roof = {"bound": "mfma", "pipe": "fp32 valu (FFT + pointwise; no dense contraction)",
             "kernel": f"k_tangent<{N},{T}>", "achieved": round(achieved, 3), "peak": peak,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-            "traffic": MEASURED_TRAFFIC_BYTES.get(f"k_tangent<{N},{T}>"),
+            "traffic": MEASURED_TRAFFIC_BYTES.get((f"k_tangent<{N},{T}>", mixed["schedule"]["steps_per_launch"])),
             "flops_per_step": fl / args.steps, "kernel_s_per_step": mixed["chain_ms"] / 1e3 / args.steps,
             "avg_launch_ms": mixed["chain_ms"] / max(mixed["launches"], 1),
             "launches_per_step": mixed["launches"] / args.steps}

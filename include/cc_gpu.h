@@ -108,6 +108,19 @@ cc_status cc_newton_step(cc_ctx *ctx, const double *x, double *F0, double *dx);
 cc_status cc_newton_step_device_async(cc_ctx *ctx, const double *d_x, double *d_F0, double *d_dx);
 cc_status cc_synchronize(cc_ctx *ctx);
 
+/* The loop of NewtonSolver::run (NewtonSolver.cpp:279-366) with the
+ * mismatch norm, the damping min(1, slow_error/err), the update and the
+ * divergence guard on the device (one pinned status word read per
+ * iteration, no host copies of F0 or dx): up to max_iter Newton steps from
+ * x.  outcome 1: converged (err < precision_newton), x is that point;
+ * outcome 2: the mismatch rose five times in a row, x is the last point whose
+ * mismatch fell; outcome 0: max_iter steps without either, x is the last
+ * iterate.  *iterations is the reference's `its` at termination, *mismatch the
+ * final err (errOld for outcome 2), history (may be null, max_iter doubles)
+ * the mismatch of each step.  On an error status x is left unchanged. */
+cc_status cc_newton_run(cc_ctx *ctx, double *x, double precision_newton, double slow_error, int32_t max_iter,
+                        int32_t *iterations, double *mismatch, double *history, int32_t *outcome);
+
 /* Copies the Jacobian assembled by the last cc_jacobian / cc_newton_step*
  * call (n x n row-major) to host memory: the matrix the reference writes as
  * jacobian_<Dim>_<i>.csv in Debug runs (NewtonSolver.cpp:337-341). */
@@ -161,7 +174,7 @@ enum { CC_TUNE_THREADS = 1, CC_TUNE_STAGES = 2 };
 /* Autotuner.  CC_TUNE_THREADS: times every thread count of each chain
  * kernel on a slice of the integration at x (device work only, a few
  * hundred ms).  CC_TUNE_STAGES: times whole Newton steps at x for
- * steps_per_launch in {128, 256, 512, 1024} and, in CC_MIXED, pipeline on/off.
+ * steps_per_launch in {64, 128, 256, 512, 1024} and, in CC_MIXED, pipeline on/off.
  * Installs and returns the fastest schedule.  times (may be NULL) receives
  * up to ntimes candidate timings in ms, in the order tried; *ntried their
  * count. */

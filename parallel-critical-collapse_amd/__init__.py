@@ -119,6 +119,7 @@ def load():
     L.cc_profile.argtypes = [_vp, C.c_int32]
     L.cc_chain_timing.argtypes = [_vp, _dp, _ip, _lp, _lp]
     L.cc_shoot_fields.argtypes = [_vp, _dp, C.c_int32, C.c_int32, _ip, _dp, _dp]
+    L.cc_newton_run.argtypes = [_vp, _dp, C.c_double, C.c_double, C.c_int32, _ip, _dp, _dp, _ip]
     L.cc_pack.argtypes = [_vp, _dp, _dp, C.c_int32]
     L.cc_unpack.argtypes = [_vp, _dp, _dp, C.c_int32]
     _pp = C.POINTER(_dp)
@@ -192,6 +193,18 @@ class Context:
         J = np.zeros((self.n, self.n))
         self._check(load().cc_last_jacobian(self.h, _d(J)), "cc_last_jacobian")
         return J
+
+    def newton_run(self, x, precision_newton, slow_error, max_iter):
+        """cc_newton_run: the whole Newton loop on the device.  Returns
+        (x, iterations, mismatch, history, outcome)."""
+        x = _arr(x).copy()
+        its, out = C.c_int32(0), C.c_int32(0)
+        err = C.c_double(0.0)
+        hist = np.zeros(max_iter)
+        self._check(load().cc_newton_run(self.h, _d(x), precision_newton, slow_error, int(max_iter), C.byref(its),
+                                         C.byref(err), _d(hist), C.byref(out)), "cc_newton_run")
+        nh = min(max_iter, its.value + 1) if out.value else max_iter
+        return x, its.value, err.value, hist[:nh].copy(), out.value
 
     def shoot_fields(self, x, every):
         """Debug field snapshots of the base shoot (cc_shoot_fields): grid

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -75,6 +76,7 @@ struct StatusBlock {
     int32_t *tstatus;            // tangent chains
     int64_t *tits;
     int32_t *info;               // LU
+    int32_t *newton;             // device Newton loop: [0] outcome of the last update
 };
 
 struct cc_ctx {
@@ -96,6 +98,8 @@ struct cc_ctx {
     double *A = nullptr;                 // n x n LU work (FP64)
     float *A32 = nullptr;                // n x n LU work (FP32)
     int32_t *perm = nullptr, *lu_swap = nullptr, *info = nullptr;   // LU row permutation, panel swap list
+    double *xold = nullptr, *nst = nullptr;  // device Newton loop: last improving x; err, errOld
+    int32_t *nist = nullptr;                 // device Newton loop: rises, outcome
     ColumnBuffers main;                  // base + every column (FP64 Jacobian)
     ColumnBuffers base;                  // base + Delta column (residual, mixed policy)
     Plan main_plan, base_plan;
@@ -746,7 +750,10 @@ void drop_graph(cc_ctx *c)
     c->gkey[0] = c->gkey[1] = c->gkey[2] = nullptr;
 }
 
-cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double *d_dx)
+// post (may be null): more work enqueued on s0 behind the step, before the
+// step's one host synchronisation.
+cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double *d_dx,
+                             const std::function<cc_status()> *post = nullptr)
 {
     c->counters[0] = c->counters[1] = c->counters[2] = 0;
     time_reset(c);
@@ -780,6 +787,7 @@ cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double 
         CCCHK(enqueue_newton(c, d_x, d_F0, d_dx));
         c->prof_pending = c->profile;
     }
+    if (post) CCCHK((*post)());
     HIPCHK(c, hipStreamSynchronize(c->s0));
     const cc_status st = tangent_policy(c) ? eval_tangent(c, nullptr) : eval_fp64(c, nullptr);
     CCCHK(st);
@@ -911,7 +919,7 @@ cc_status tune_stages(cc_ctx *c, std::vector<double> &times)
     double best_ms = 1e300;
     for (int pipe : {1, 0}) {
         if (pipe && !mixed) continue;
-        for (int S : {128, 256, 512, 1024}) {
+        for (int S : {64, 128, 256, 512, 1024}) {
             if ((max_chain_steps(c) + S - 1) / S > kMaxChunks) continue;
             drop_graph(c);
             c->sched.steps_per_launch = S;
@@ -1261,6 +1269,9 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
     ALLOC(c->A, (size_t)n * n);
     ALLOC(c->A32, (size_t)n * n);
     ALLOC(c->perm, n);
+    ALLOC(c->xold, n);
+    ALLOC(c->nst, 2);
+    ALLOC(c->nist, 4);
     ALLOC(c->lu_swap, 1 + 2 * ccd::LU_SWAP_MAX);
     ALLOC(c->info, 1);
     for (ColumnBuffers *B : {&c->main, &c->base}) {
@@ -1298,7 +1309,7 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
     // pinned status block
     {
         const size_t mch = c->maxchains, mc = c->maxcols;
-        const size_t bytes = sizeof(int64_t) * (mch + 4 + (size_t)n) + sizeof(int32_t) * (mch + mc + 4 + 2 + n + 2);
+        const size_t bytes = sizeof(int64_t) * (mch + 4 + (size_t)n) + sizeof(int32_t) * (mch + mc + 4 + 2 + n + 2 + 4);
         if (bad(hipHostMalloc(&c->hblock, bytes, hipHostMallocDefault), "hipHostMalloc")) return CC_DEVICE;
         char *q = static_cast<char *>(c->hblock);
         auto take = [&](auto **ptr, size_t cnt) {
@@ -1314,6 +1325,7 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
         take(&c->hs.bmstatus, 2);
         take(&c->hs.tstatus, n);
         take(&c->hs.info, 2);
+        take(&c->hs.newton, 4);
     }
     // plans to the device
     if (bad(upload_plan(c->main_plan, c->main), "plan")) return CC_DEVICE;
@@ -1585,6 +1597,57 @@ cc_status cc_newton_step(cc_ctx *c, const double *x, double *F0, double *dx)
     HIPCHK(c, hipMemcpyAsync(dx, c->dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));
     return CC_OK;
+}
+
+cc_status cc_newton_run(cc_ctx *c, double *x, double precision_newton, double slow_error, int32_t max_iter,
+                        int32_t *iterations, double *mismatch, double *history, int32_t *outcome)
+{
+    if (!c || !x || max_iter < 1 || !iterations || !mismatch || !outcome) return CC_BADARG;
+    const int n = c->n;
+    double *hist = nullptr;
+    HIPCHK(c, hipMalloc(&hist, sizeof(double) * max_iter));
+    auto done = [&](cc_status st) {
+        (void)hipFree(hist);
+        return st;
+    };
+    const double st0[2] = {1.0, 1.0};
+    const int32_t ist0[4] = {0, 0, 0, 0};
+    hipError_t e = hipMemcpyAsync(c->x, x, sizeof(double) * n, hipMemcpyHostToDevice, c->s0);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->xold, c->x, sizeof(double) * n, hipMemcpyDeviceToDevice, c->s0);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->nst, st0, sizeof(st0), hipMemcpyHostToDevice, c->s0);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->nist, ist0, sizeof(ist0), hipMemcpyHostToDevice, c->s0);
+    if (e != hipSuccess) {
+        c->err = std::string("hip: ") + hipGetErrorString(e);
+        return done(CC_DEVICE);
+    }
+    int its = 0, mode = 0;
+    for (its = 0; its < max_iter; ++its) {
+        const std::function<cc_status()> post = [&]() -> cc_status {
+            hipLaunchKernelGGL(ccd::k_newton_update, dim3(1), dim3(256), 0, c->s0, c->x, c->xold, c->F0, c->dx, n,
+                               precision_newton, slow_error, c->nst, c->nist, hist, its);
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipMemcpyAsync(c->hs.newton, c->nist + 1, sizeof(int32_t), hipMemcpyDeviceToHost, c->s0));
+            return CC_OK;
+        };
+        const cc_status st = newton_step_device(c, c->x, c->F0, c->dx, &post);
+        if (st != CC_OK) return done(st);
+        mode = c->hs.newton[0];
+        if (mode != 0) break;
+    }
+    double stv[2] = {0.0, 0.0};
+    const int nh = std::min(max_iter, its + 1);
+    e = hipMemcpyAsync(x, mode == 2 ? c->xold : c->x, sizeof(double) * n, hipMemcpyDeviceToHost, c->s0);
+    if (e == hipSuccess) e = hipMemcpyAsync(stv, c->nst, sizeof(stv), hipMemcpyDeviceToHost, c->s0);
+    if (e == hipSuccess && history) e = hipMemcpyAsync(history, hist, sizeof(double) * nh, hipMemcpyDeviceToHost, c->s0);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s0);
+    if (e != hipSuccess) {
+        c->err = std::string("hip: ") + hipGetErrorString(e);
+        return done(CC_DEVICE);
+    }
+    *outcome = mode;
+    *iterations = mode == 0 ? max_iter : its;
+    *mismatch = mode == 2 ? stv[1] : stv[0];
+    return done(CC_OK);
 }
 
 cc_status cc_newton_step_device_async(cc_ctx *c, const double *d_x, double *d_F0, double *d_dx)
@@ -2196,7 +2259,7 @@ cc_status cc_batch_autotune(cc_batch *b, const double *const *x, double *times, 
         double best_ms = 1e300;
         for (int T : Ts)
             for (int pipe : {1, 0})
-                for (int S : {128, 512}) {
+                for (int S : {64, 128, 512}) {
                     if ((C.maxsteps + S - 1) / S > kMaxChunks) continue;
                     C.sched.threads_tangent = T;
                     C.sched.pipeline = pipe;

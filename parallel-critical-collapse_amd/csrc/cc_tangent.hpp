@@ -78,6 +78,12 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
     using RC = Rec<N>;
     const R half = R(0.5), invN = R(1.0) / (R)N;
     asm volatile("" : "+v"(t));
+    // exchange regions of the chained transforms (as rhs_pair): no trailing
+    // barriers; the caller's reduction barrier ends the evaluation
+    constexpr int DB = tangent_db<N, R>(), RS = 2 * N;
+    constexpr int pA = 0, pB = DB ? fft_next_par<N, T>(pA) : 0, pC = DB ? fft_next_par<N, T>(pB) : 0;
+    constexpr int pS = DB ? fft_next_par<N, T>(pC) : 0, pG = DB ? 1 - pS : 0;
+    constexpr int pH = DB ? fft_next_par<N, T>(pG) : 0;
     // per-stage products that survive to the pointwise tangent RHS
     // (the base-only products B_U, B_V, C_F are recomputed from the record in
     // the pointwise stage: held across the transforms they would spill)
@@ -114,7 +120,8 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             const cx<R> v = z[0][p];
             z[1][p] = {w * v.y, -(w * v.x)};
         }
-        fft_block<N, T, -1, 2, R, tangent_db<N, R>()>(z, xb, tw, t);
+        if (s == 0) fft_block<N, T, -1, 2, R, DB, pA, 0, RS>(z, xb, tw, t);
+        else fft_block<N, T, -1, 2, R, DB, pB, 0, RS>(z, xb, tw, t);
         // The record of a step is the same for every fixed-point iteration and
         // every transform; an opaque offset keeps this stage's loads behind its
         // transform instead of hoisted in front of it (or of the loop), where
@@ -156,16 +163,16 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         }
     }
     // dIA2 = solveInhom with the base's integrating factor and lambda
-    fft1<N, T, +1, R, tangent_db<N, R>()>(c, xb, tw, t);
+    fft1<N, T, +1, R, DB, pC, 0, RS>(c, xb, tw, t);
 #pragma unroll
     for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
 #pragma unroll
-    for (int p = 0; p < P; ++p) xb[t + T * p] = c[p];
+    for (int p = 0; p < P; ++p) xb[pS * RS + t + T * p] = c[p];
     __syncthreads();
     cx<R> cm[P];
 #pragma unroll
-    for (int p = 0; p < P; ++p) cm[p] = xb[(N - (t + T * p)) & (N - 1)];
-    __syncthreads();
+    for (int p = 0; p < P; ++p) cm[p] = xb[pS * RS + ((N - (t + T * p)) & (N - 1))];
+    if constexpr (!DB) __syncthreads();
     const R lam0 = rec[RC::LAM], lam1 = rec[RC::STAGE + RC::LAM];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -193,7 +200,7 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         }
         c[p] = {G[0].x - G[1].y, G[0].y + G[1].x};
     }
-    fft1<N, T, -1, R, tangent_db<N, R>()>(c, xb, tw, t);
+    fft1<N, T, -1, R, DB, pG, 0, RS>(c, xb, tw, t);
     // pointwise tangent RHS
     cx<R> yp[2][P];
 #pragma unroll
@@ -221,7 +228,7 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             yp[s][p] = {dUx, dVx + dFx};
         }
     }
-    fft_block<N, T, +1, 2, R, tangent_db<N, R>()>(yp, xb, tw, t);
+    fft_block<N, T, +1, 2, R, DB, pH, 0, RS>(yp, xb, tw, t);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -372,14 +379,18 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
 #pragma unroll
         for (int h = 0; h < H; ++h) ys[0][t + T * h] = ys[1][t + T * h];
     }
+    // addresses of the final stores are formed here, not kept (spilled)
+    // across the step loop from the prologue
+    int te = t;
+    asm volatile("" : "+v"(te));
 #pragma unroll
-    for (int h = 0; h < H; ++h) Yg[t + T * h] = ys[0][t + T * h];
+    for (int h = 0; h < H; ++h) Yg[te + T * h] = ys[0][te + T * h];
     if (PRED && a.fprev && have_prev) {
         cx<R> *fp = a.fprev + (size_t)ch * 2 * M;
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            fp[t + T * h] = fs[0][t + T * h];
-            fp[M + t + T * h] = fs[1][t + T * h];
+            fp[te + T * h] = fs[0][te + T * h];
+            fp[M + te + T * h] = fs[1][te + T * h];
         }
     }
     if (t == 0) {

@@ -3,8 +3,8 @@
 //
 //   cc_newton [-s | -m] [-r] [-i input.json] [-o output.json]
 //             [--precision mixed|fp64] [--autotune] [--graph] [--reuse-sides]
-//             [--ignore-converged] [--benchmark] [--benchmark-repetitions K]
-//             [--quiet] [--dry-run]
+//             [--device-loop] [--ignore-converged] [--benchmark]
+//             [--benchmark-repetitions K] [--quiet] [--dry-run]
 //
 // Single run (-s, default): the result dictionary (NewtonSolver.cpp:775-816)
 // is written to output.json, by default back into input.json as the reference
@@ -14,7 +14,9 @@
 // whole suite is rewritten after each one (main.cpp:236-378).  Benchmark: K
 // repetitions of one simulation, per-repetition NewtonStep_<i> and
 // OverallTime, written to benchmark_D=<Dim>_GPU_<policy>.json next to the
-// input (main.cpp:136-201).  --dry-run parses the input and prints it back; with
+// input (main.cpp:136-201).  --device-loop runs the Newton loop's norm, damping
+// and update on the device (cc_newton_run; the host loop stays in use for
+// Debug output and benchmarks).  --dry-run parses the input and prints it back; with
 // -m it prints the run order and where each dimension's initial data would come
 // from, and -o receives the suite with the initial data filled in.
 #include <cstdio>
@@ -40,8 +42,8 @@ void write_json(const std::string &path, const Json &j, int indent = 4)
 int usage()
 {
     std::cerr << "usage: cc_newton [-s | -m] [-r] [-i input.json] [-o output.json] [--precision mixed|fp64]\n"
-                 "                 [--autotune] [--graph] [--reuse-sides] [--ignore-converged] [--benchmark]\n"
-                 "                 [--benchmark-repetitions K] [--quiet] [--dry-run]\n";
+                 "                 [--autotune] [--graph] [--reuse-sides] [--device-loop] [--ignore-converged]\n"
+                 "                 [--benchmark] [--benchmark-repetitions K] [--quiet] [--dry-run]\n";
     return EXIT_FAILURE;
 }
 
@@ -115,6 +117,7 @@ int main(int argc, char *argv[])
             } else if (a == "--autotune") opt.autotune = true;
             else if (a == "--graph") opt.use_graph = true;
             else if (a == "--reuse-sides") opt.reuse_sides = true;
+            else if (a == "--device-loop") opt.device_loop = true;
             else if (a == "--ignore-converged") ignore_converged = true;
             else if (a == "--benchmark" || a == "-b") benchmark = true;
             else if (a == "--benchmark-repetitions") {

@@ -301,6 +301,37 @@ Json NewtonSolver::run(Json *bench)
             check(cc_set_schedule(ctx_, &s), "cc_set_schedule");
         }
         bool converged = false;
+        if (opt_.device_loop && !cfg_.Debug && !bench) {
+            // the same loop with its bookkeeping on the device (cc_newton_run)
+            int32_t its = 0, outcome = 0;
+            double mis = 0.0;
+            history_.assign((size_t)cfg_.MaxIterNewton, 0.0);
+            check(cc_newton_run(ctx_, x_.data(), cfg_.PrecisionNewton, cfg_.SlowError, cfg_.MaxIterNewton, &its,
+                                &mis, history_.data(), &outcome),
+                  "Newton loop");
+            history_.resize(outcome ? (size_t)its + 1 : (size_t)cfg_.MaxIterNewton);
+            for (size_t k = 0; k < history_.size(); ++k) {
+                say("Newton iteration: " + std::to_string(k + 1));
+                if (opt_.verbose) std::cout << "Mismatch norm: " << history_[k] << std::endl;
+            }
+            if (outcome == 0) {
+                std::ostringstream m;
+                m << "Newton method did not converge in " << cfg_.MaxIterNewton << " iterations. For dimension: "
+                  << cfg_.Dim;
+                throw SolverError(CC_IRK_NOCONV, m.str());
+            }
+            if (outcome == 1) say("The solution has converged!\n");
+            else std::cerr << "Mismatch increased – terminating Newton.\n";
+            Delta_ = x_[dslot];
+            std::vector<double> xv = x_;
+            xv[dslot] = 0.0;
+            unpack(xv);
+            its_ = (size_t)its;
+            err_ = mis;
+            write_final_output(its_, err_);
+            if (bench) (*bench)["OverallTime"] = now_s() - t_all;
+            return result_;
+        }
         for (size_t its = 0; its < (size_t)cfg_.MaxIterNewton; ++its) {
             const double t_step = now_s();
             say("Newton iteration: " + std::to_string(its + 1));

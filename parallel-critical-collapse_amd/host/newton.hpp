@@ -44,6 +44,8 @@ struct SolverOptions {
     bool autotune = false;     // cc_autotune at the initial point before the first step
     bool use_graph = false;    // replay each Newton step from a captured hipGraph
     bool reuse_sides = false;  // FP64: reuse the base shoot's unchanged side
+    bool device_loop = false;  // norm, damping and update on the device (cc_newton_run); not with
+                               // Debug output or benchmark timings, which need the host loop
     bool verbose = true;       // the reference's progress lines on stdout
     std::string debug_dir = "."; // where Debug runs write jacobian_<Dim>_<i>.csv
 };

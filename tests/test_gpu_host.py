@@ -155,3 +155,29 @@ def test_cc_newton_multiple_run(anchored_config, tmp_path):
     # first three predicts the fourth to far better than the step between them
     c = np.polyfit([4.00, 4.01, 4.02], deltas[:3], 2)
     assert abs(np.polyval(c, 4.03) - deltas[3]) < 0.1 * abs(deltas[3] - deltas[2])
+
+
+def test_device_newton_loop_matches_host_loop(anchored_config):
+    """cc_newton_run (norm, damping and update on the device) takes the same
+    steps as the host loop over cc_newton_step: the same iterate, iteration
+    count and mismatch history, bit for bit."""
+    shape, _, d, x0 = anchored_config
+    ctx = cc.Context(cc.Problem.from_shape(shape, precision=cc.CC_MIXED))
+    print("device Newton loop", flush=True)
+    x, its, err, hist, outcome = ctx.newton_run(x0, d["PrecisionNewton"], d["SlowError"], d["MaxIterNewton"])
+    xr, its_r, err_r, hist_r, _ = newton_loop(shape, x0, d["PrecisionNewton"], d["MaxIterNewton"], d["SlowError"])
+    assert outcome == 1 and its == its_r and err == err_r
+    assert np.array_equal(hist, hist_r)
+    assert np.array_equal(x, xr)
+
+
+def test_cc_newton_device_loop_writes_the_same_result(anchored_config, tmp_path):
+    shape, p, d, _ = anchored_config
+    outs = []
+    for flags in ([], ["--device-loop"]):
+        out = tmp_path / f"res{len(flags)}.json"
+        r = subprocess.run([CLI, "-i", str(p), "-o", str(out), "--quiet", *flags], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(json.loads(out.read_text()))
+    assert outs[0] == outs[1]
