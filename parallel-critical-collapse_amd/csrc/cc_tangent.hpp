@@ -201,31 +201,35 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         c[p] = {G[0].x - G[1].y, G[0].y + G[1].x};
     }
     fft1<N, T, -1, R, DB, pG, 0, RS>(c, xb, tw, t);
-    // pointwise tangent RHS
+    // pointwise tangent RHS: the half-period products of point jh serve both
+    // points jh and jh + N/2 (U, V odd and F even under the half shift)
     cx<R> yp[2][P];
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-        const int ph = (p < H) ? p : p - H;
-        const R sg = (p < H) ? R(1.0) : -R(1.0);
-        const int j = t + T * p;
+    for (int h = 0; h < H; ++h) {
+        const int jh = t + T * h;
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             int fin_off = s * RC::STAGE;
             asm volatile("" : "+s"(fin_off));          // behind the inverse transform, as above
             const R *rs = rec + fin_off;
-            const R r = rs[RC::R + j], E = rs[RC::E + j];
-            const int jh = t + T * ph;
             const R U = rs[RC::U + jh], V = rs[RC::V + jh], F = rs[RC::F + jh], FmX = rs[RC::FMX + jh];
             const R X = xs[s], two_x = R(2.0) * X;
             const R BU = K.two_dm3 * F * U * trcp(two_x * (F + X));
             const R BV = K.two_dm3 * F * V * trcp(two_x * FmX);
             const R CF = K.dm3 * F * trcp(X);
-            const R dIA2 = (s ? c[p].y : c[p].x) * trcp(E);
-            const R dr = -r * r * dIA2;
-            const R dUx = sg * (dAU[s][ph] - dBU[s][ph] * r - BU * dr);
-            const R dVx = sg * (dAV[s][ph] - dBV[s][ph] * r - BV * dr);
-            const R dFx = dCF[s][ph] * (r - R(1.0)) + CF * dr;
-            yp[s][p] = {dUx, dVx + dFx};
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int p = h + half * H;
+                const int j = t + T * p;
+                const R sg = half ? -R(1.0) : R(1.0);
+                const R r = rs[RC::R + j], E = rs[RC::E + j];
+                const R dIA2 = (s ? c[p].y : c[p].x) * trcp(E);
+                const R dr = -r * r * dIA2;
+                const R dUx = sg * (dAU[s][h] - dBU[s][h] * r - BU * dr);
+                const R dVx = sg * (dAV[s][h] - dBV[s][h] * r - BV * dr);
+                const R dFx = dCF[s][h] * (r - R(1.0)) + CF * dr;
+                yp[s][p] = {dUx, dVx + dFx};
+            }
         }
     }
     fft_block<N, T, +1, 2, R, DB, pH, 0, RS>(yp, xb, tw, t);
