@@ -48,7 +48,8 @@ typedef enum cc_precision {
  * (include/SimulationConfig.hpp:73-137) plus the precision policy. */
 typedef struct cc_problem {
     int32_t Ntau;           /* 128, 256, 512, 1024 or 2048 */
-    int32_t SchemeIRK;      /* 2 (Gauss-Legendre 2 stages; the only scheme on the device) */
+    int32_t SchemeIRK;      /* Gauss-Legendre stages 1, 2 or 3 (ODEStepper.cpp:22-80); the
+                               tangent-linear policies (CC_MIXED, CC_FP64_TANGENT) need 2 */
     int32_t MaxIterIRK;
     int32_t NLeft, NRight;
     double  Dim;

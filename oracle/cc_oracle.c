@@ -121,6 +121,16 @@ void cco_destroy(cco_ctx *c)
     free(c);
 }
 
+int cco_tableau(const cco_ctx *c, double *a, double *b, double *cnode)
+{
+    for (int i = 0; i < c->stages; ++i) {
+        b[i] = c->b[i];
+        cnode[i] = c->cnode[i];
+        for (int k = 0; k < c->stages; ++k) a[i * c->stages + k] = c->a[i][k];
+    }
+    return c->stages;
+}
+
 void cco_counters(const cco_ctx *c, int64_t out[3])
 {
     out[0] = c->n_steps; out[1] = c->n_its; out[2] = c->n_rhs;

@@ -58,6 +58,8 @@ void     cco_destroy(cco_ctx *c);
 /* Counters since the last reset: IRK steps taken, fixed-point iterations
  * (sum of itsReached, ODEStepper.cpp:202-204), RHS evaluations. */
 void cco_counters(const cco_ctx *c, int64_t out[3]);
+/* the Butcher tableau in use: returns the stage count s; a (s x s, row-major), b, c */
+int cco_tableau(const cco_ctx *c, double *a, double *b, double *cnode);
 void cco_reset_counters(cco_ctx *c);
 
 /* --- SpectralTransformer (SpectralTransformer.cpp) ---------------------- */

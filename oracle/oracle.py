@@ -56,6 +56,8 @@ def lib():
         L.cco_create.argtypes = [C.POINTER(Problem)]
         L.cco_destroy.argtypes = [C.c_void_p]
         L.cco_counters.argtypes = [C.c_void_p, _lp]
+        L.cco_tableau.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.cco_tableau.restype = C.c_int
         L.cco_reset_counters.argtypes = [C.c_void_p]
         for name in ("cco_forward_r2c", "cco_backward_c2r", "cco_forward_c2c", "cco_backward_c2c"):
             getattr(L, name).argtypes = [C.c_void_p, _dp, _dp]
@@ -144,6 +146,12 @@ class Oracle:
         out = np.zeros(3, np.int64)
         lib().cco_counters(self.ctx, out.ctypes.data_as(_lp))
         return out
+
+    def tableau(self):
+        """(a, b, c) of the Gauss-Legendre scheme in use."""
+        a, b, c = np.zeros(9), np.zeros(3), np.zeros(3)
+        s = lib().cco_tableau(self.ctx, _p(a), _p(b), _p(c))
+        return a[:s * s].reshape(s, s), b[:s], c[:s]
 
     def reset_counters(self):
         lib().cco_reset_counters(self.ctx)

@@ -87,7 +87,8 @@ struct cc_ctx {
     std::vector<hipEvent_t> chunk_ev;    // base chunk k done (pipelined mixed step)
     std::string err;
     int64_t counters[3] = {0, 0, 0};
-    double a[2][2], b[2], c[2];
+    int stages = 2;                        // Gauss-Legendre stages (SchemeIRK)
+    double a[3][3] = {}, b[3] = {}, c[3] = {};
     double *grid = nullptr;
     cx<double> *tw64 = nullptr;
     cx<float> *tw32 = nullptr;
@@ -353,6 +354,17 @@ cc_status ensure_chunk_events(cc_ctx *c, int nchunks)
 // ---------------------------------------------------------------------------
 // Column launches (FP64)
 // ---------------------------------------------------------------------------
+template <typename R>
+void set_tableau(ccd::ChainParams<R> &a, const cc_ctx *c)
+{
+    for (int i = 0; i < 3; ++i) {
+        a.b[i] = c->b[i];
+        a.c[i] = c->c[i];
+        for (int k = 0; k < 3; ++k) a.a[i][k] = c->a[i][k];
+    }
+    a.stages = c->stages;
+}
+
 template <int N>
 ccd::ChainParams<double> chain_params(const cc_ctx *c, const ColumnBuffers &B)
 {
@@ -371,14 +383,7 @@ ccd::ChainParams<double> chain_params(const cc_ctx *c, const ColumnBuffers &B)
     a.maxIts = c->p.MaxIterIRK;
     a.Dim = c->p.Dim;
     a.precision = c->p.PrecisionIRK;
-    a.a00 = c->a[0][0];
-    a.a01 = c->a[0][1];
-    a.a10 = c->a[1][0];
-    a.a11 = c->a[1][1];
-    a.b0 = c->b[0];
-    a.b1 = c->b[1];
-    a.c0 = c->c[0];
-    a.c1 = c->c[1];
+    set_tableau(a, c);
     a.rec = nullptr;
     a.rec_base = nullptr;
     return a;
@@ -813,7 +818,7 @@ cc_status tune_threads_n(cc_ctx *c, const double *d_x, std::vector<double> &time
     const bool mixed = c->p.precision == CC_MIXED;
     const bool t64 = c->p.precision == CC_FP64_TANGENT;     // FP64 record and tangent: default threads only
     std::vector<int> Ts = {O::kThreads};
-    if (O::kAltThreads && !t64) Ts.push_back(O::kAltThreads);
+    if (O::kAltThreads && !t64 && c->stages == 2) Ts.push_back(O::kAltThreads);
     hipStream_t s = c->s0;
     const int maxsteps = max_chain_steps(c);
     auto pick = [&](int &slot, const std::vector<double> &t) {
@@ -1182,7 +1187,9 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
     *out = nullptr;
     const int N = p->Ntau;
     if (!(N == 128 || N == 256 || N == 512 || N == 1024 || N == 2048)) return CC_BADARG;
-    if (p->SchemeIRK != 2 || p->NLeft < 1 || p->NRight < 1 || p->MaxIterIRK < 1) return CC_BADARG;
+    if (p->SchemeIRK < 1 || p->SchemeIRK > 3 || p->NLeft < 1 || p->NRight < 1 || p->MaxIterIRK < 1) return CC_BADARG;
+    // the tangent-linear columns and the batch are built for the two-stage scheme
+    if (p->SchemeIRK != 2 && p->precision != CC_FP64) return CC_BADARG;
     if (p->precision != CC_FP64 && p->precision != CC_MIXED && p->precision != CC_FP64_TANGENT) return CC_BADARG;
     if (!(p->XLeft > 0.0 && p->XLeft < p->XMid && p->XMid < p->XRight && p->XRight < 1.0)) return CC_BADARG;
     cc_ctx *c = new cc_ctx();
@@ -1197,14 +1204,37 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
     c->X = p->NLeft + p->NRight;
     c->iM = p->NLeft;
     c->iR = c->X;
-    // Gauss-Legendre s=2 tableau (ODEStepper.cpp:47-54)
-    c->a[0][0] = 0.25;
-    c->a[0][1] = 0.25 - 0.5 / std::sqrt(3.0);
-    c->a[1][0] = 0.25 + 0.5 / std::sqrt(3.0);
-    c->a[1][1] = 0.25;
-    c->b[0] = c->b[1] = 0.5;
-    c->c[0] = 0.5 - 0.5 / std::sqrt(3.0);
-    c->c[1] = 0.5 + 0.5 / std::sqrt(3.0);
+    // Gauss-Legendre tableaus (ODEStepper.cpp:29-78)
+    c->stages = p->SchemeIRK;
+    if (c->stages == 1) {
+        c->a[0][0] = 0.5;
+        c->b[0] = 1.0;
+        c->c[0] = 0.5;
+    } else if (c->stages == 2) {
+        c->a[0][0] = 0.25;
+        c->a[0][1] = 0.25 - 0.5 / std::sqrt(3.0);
+        c->a[1][0] = 0.25 + 0.5 / std::sqrt(3.0);
+        c->a[1][1] = 0.25;
+        c->b[0] = c->b[1] = 0.5;
+        c->c[0] = 0.5 - 0.5 / std::sqrt(3.0);
+        c->c[1] = 0.5 + 0.5 / std::sqrt(3.0);
+    } else {
+        const double r15 = std::sqrt(15.0);
+        c->a[0][0] = 5.0 / 36.0;
+        c->a[0][1] = 2.0 / 9.0 - 1.0 / r15;
+        c->a[0][2] = 5.0 / 36.0 - 0.5 / r15;
+        c->a[1][0] = 5.0 / 36.0 + r15 / 24.0;
+        c->a[1][1] = 2.0 / 9.0;
+        c->a[1][2] = 5.0 / 36.0 - r15 / 24.0;
+        c->a[2][0] = 5.0 / 36.0 + 0.5 / r15;
+        c->a[2][1] = 2.0 / 9.0 + 1.0 / r15;
+        c->a[2][2] = 5.0 / 36.0;
+        c->b[0] = c->b[2] = 5.0 / 18.0;
+        c->b[1] = 4.0 / 9.0;
+        c->c[0] = 0.5 - r15 / 10.0;
+        c->c[1] = 0.5;
+        c->c[2] = 0.5 + r15 / 10.0;
+    }
     // default schedule
     c->sched.threads_base = c->sched.threads_bulk = c->sched.threads_tangent = ccd::default_threads(N);
     c->sched.steps_per_launch = kDefaultStepsPerLaunch;
@@ -1474,8 +1504,10 @@ cc_status cc_set_schedule(cc_ctx *c, const cc_schedule *s)
     if (!valid_threads(c->N, s->threads_base) || !valid_threads(c->N, s->threads_bulk) ||
         !valid_threads(c->N, s->threads_tangent) ||
         (c->p.precision == CC_FP64_TANGENT && (s->threads_base != ccd::default_threads(c->N) ||
-                                               s->threads_tangent != ccd::default_threads(c->N)))) {
-        c->err = "unsupported thread count for this Ntau and precision";
+                                               s->threads_tangent != ccd::default_threads(c->N))) ||
+        (c->stages != 2 && (s->threads_base != ccd::default_threads(c->N) ||
+                            s->threads_bulk != ccd::default_threads(c->N)))) {
+        c->err = "unsupported thread count for this Ntau, precision and scheme";
         return CC_BADARG;
     }
     if (s->steps_per_launch < 16 || s->steps_per_launch > 8192 ||
@@ -1858,6 +1890,10 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
                        const int32_t *end, int32_t batch, int32_t fp32, int64_t *its_total, int32_t *status)
 {
     if (!c || !Y || !grid || ngrid < 2 || !start || !end || batch < 1) return CC_BADARG;
+    if (fp32 && c->stages != 2) {
+        c->err = "FP32 chains integrate the two-stage scheme only";
+        return CC_BADARG;
+    }
     const int N = c->N, M = c->M;
     std::vector<int32_t> ns(batch), dr(batch);
     std::vector<double> dl(batch);
@@ -1913,14 +1949,7 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
             a.maxIts = c->p.MaxIterIRK;
             a.Dim = c->p.Dim;
             a.precision = sizeof(R) == 8 ? c->p.PrecisionIRK : c->p.PrecisionIRK32;
-            a.a00 = c->a[0][0];
-            a.a01 = c->a[0][1];
-            a.a10 = c->a[1][0];
-            a.a11 = c->a[1][1];
-            a.b0 = c->b[0];
-            a.b1 = c->b[1];
-            a.c0 = c->c[0];
-            a.c1 = c->c[1];
+            set_tableau(a, c);
             const int S = c->sched.steps_per_launch;
             const int T = sizeof(R) == 8 ? c->sched.threads_base : O::kThreads;
             for (int s0 = 0; s0 < maxsteps && e == hipSuccess; s0 += S) {

@@ -53,7 +53,8 @@ struct ChainParams {
     int32_t maxIts;
     double Dim;
     double precision;
-    double a00, a01, a10, a11, b0, b1, c0, c1;   // Butcher tableau
+    double a[3][3], b[3], c[3];    // Gauss-Legendre tableau (ODEStepper.cpp:22-80), stages x stages
+    int32_t stages;                // 1, 2 or 3
     const double *dim;             // per-chain D (batched problems), null: Dim for every chain
     const int32_t *goff;           // per-chain offset of its grid in `grid`, null: 0
     void *rec;                     // base-trajectory record (tangent-linear columns), may be null;
@@ -302,10 +303,11 @@ __device__ __forceinline__ RhsConst<R> make_const(double Dim, double delta)
     return K;
 }
 
-// Waves per SIMD the register budget is cut for: FP64 chains at four values
-// per thread fit 2 (256 VGPRs), at eight values 1; FP32 chains fit 3.
-template <int N, int T, typename R>
-constexpr int chain_waves() { return sizeof(R) == 8 ? (N / T >= 8 ? 1 : 2) : 3; }
+// Waves per SIMD the register budget is cut for: FP64 two-stage chains at
+// four values per thread fit 2 (256 VGPRs), at eight values or with another
+// stage count 1; FP32 chains fit 3.
+template <int N, int T, typename R, int S = 2>
+constexpr int chain_waves() { return sizeof(R) == 8 ? ((N / T >= 8 || S != 2) ? 1 : 2) : 3; }
 
 // RT: element type of the base-trajectory record (float for the FP32 tangent,
 // double for the FP64 one); unused when a.rec is null.  DB = 1 double-buffers
@@ -314,10 +316,14 @@ constexpr int chain_waves() { return sizeof(R) == 8 ? (N / T >= 8 ? 1 : 2) : 3; 
 // Jacobian launches that need two workgroups per CU.  Every step starts the
 // fixed point from the reference's K_s = y (ODEStepper.cpp:193-199): F(x)
 // depends on the start at the level of the iteration tolerance, which the
-// backward integration from the horizon amplifies to ~1e-12.
-template <int N, int T, typename R, typename RT = float, int DB = 0>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N, T, R>()))) void k_chain(ChainParams<R> a)
+// backward integration from the horizon amplifies to ~1e-12.  S: stages of
+// the Gauss-Legendre scheme (SchemeIRK 1, 2, 3).  Two stages are evaluated
+// together (rhs_pair); one stage runs as a pair of equal stages, three as a
+// pair plus a pair of equal stages.  Only S = 2 records the trajectory.
+template <int N, int T, typename R, typename RT = float, int DB = 0, int S = 2>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N, T, R, S>()))) void k_chain(ChainParams<R> a)
 {
+    static_assert(S >= 1 && S <= 3, "Gauss-Legendre schemes with 1, 2 or 3 stages");
     constexpr int P = N / T, H = P / 2, M = N / 2;
     __shared__ cx<R> xb[(1 + DB) * 2 * N];
     __shared__ R bc[8];
@@ -335,7 +341,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
     // are read once per iteration: they live in LDS (thread-private slots),
     // the stage iterates in registers.
     __shared__ cx<R> ys[2][M];
-    cx<R> yk[2][H];
+    cx<R> yk[S][H];
 #pragma unroll
     for (int h = 0; h < H; ++h) ys[0][t + T * h] = Yg[t + T * h];
     const RhsConst<R> K = make_const<N, R>(a.dim ? a.dim[ch] : a.Dim, a.delta[ch]);
@@ -348,48 +354,93 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
     for (int step = s_begin; step < s_end; ++step, idx += dir) {
         const double xin = grid[idx], xout = grid[idx + dir];
         const double dxd = xout - xin;
-        const R xs[2] = {(R)(xin + dxd * a.c0), (R)(xin + dxd * a.c1)};
-        const R A00 = (R)(dxd * a.a00), A01 = (R)(dxd * a.a01), A10 = (R)(dxd * a.a10), A11 = (R)(dxd * a.a11);
-        const R B0 = (R)(dxd * a.b0), B1 = (R)(dxd * a.b1);
+        R xs[S], A[S][S], B[S];
 #pragma unroll
-        for (int h = 0; h < H; ++h) yk[0][h] = yk[1][h] = ys[0][t + T * h];
-        if (t == T - 2) bc[0] = bc[1] = yk[0][H - 1].x;   // Re of mode M-2 for the fix
+        for (int i = 0; i < S; ++i) {
+            xs[i] = (R)(xin + dxd * a.c[i]);
+            B[i] = (R)(dxd * a.b[i]);
+#pragma unroll
+            for (int k = 0; k < S; ++k) A[i][k] = (R)(dxd * a.a[i][k]);
+        }
+#pragma unroll
+        for (int h = 0; h < H; ++h)
+#pragma unroll
+            for (int i = 0; i < S; ++i) yk[i][h] = ys[0][t + T * h];
+        if (t == T - 2) {                  // Re of mode M-2 of each stage, for the fix
+            bc[0] = bc[1] = yk[0][H - 1].x;
+            if constexpr (S == 3) bc[4] = bc[5] = yk[0][H - 1].x;
+        }
         __syncthreads();
         bool conv = false;
         int its = 0;
         RT *rec = nullptr;
-        if (a.rec && a.rec_base[ch] >= 0)
+        if (S == 2 && a.rec && a.rec_base[ch] >= 0)
             rec = static_cast<RT *>(a.rec) + (size_t)(a.rec_base[ch] + step) * Rec<N>::STEP;
         for (its = 0; its < a.maxIts; ++its) {
-            cx<R> f[2][H];
-            rhs_pair<N, T, R, RT, DB>(yk, f, xs, K, a.invm, xb, bc, a.tw, t, rec);
+            cx<R> f[S][H];
+            if constexpr (S == 2) {
+                rhs_pair<N, T, R, RT, DB>(yk, f, xs, K, a.invm, xb, bc, a.tw, t, rec);
+            } else {
+                cx<R> yp[2][H], fp[2][H];
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    yp[0][h] = yk[0][h];
+                    yp[1][h] = yk[S == 3 ? 1 : 0][h];
+                }
+                const R xp[2] = {xs[0], xs[S == 3 ? 1 : 0]};
+                rhs_pair<N, T, R, RT, DB>(yp, fp, xp, K, a.invm, xb, bc, a.tw, t);
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    f[0][h] = fp[0][h];
+                    if constexpr (S == 3) f[1][h] = fp[1][h];
+                }
+                if constexpr (S == 3) {
+#pragma unroll
+                    for (int h = 0; h < H; ++h) yp[0][h] = yp[1][h] = yk[2][h];
+                    const R x2[2] = {xs[2], xs[2]};
+                    __syncthreads();       // behind the first evaluation's last exchange
+                    rhs_pair<N, T, R, RT, DB>(yp, fp, x2, K, a.invm, xb, bc + 4, a.tw, t);
+#pragma unroll
+                    for (int h = 0; h < H; ++h) f[2][h] = fp[0][h];
+                }
+            }
             R loc = (R)0;
 #pragma unroll
             for (int h = 0; h < H; ++h) {
                 // ODEStepper.cpp:241-259 (stage update, same operation order)
                 const cx<R> y = ys[0][t + T * h];
-                cx<R> n0 = y, n1 = y, nn = y;
-                n0.x += A00 * f[0][h].x; n0.y += A00 * f[0][h].y;
-                n0.x += A01 * f[1][h].x; n0.y += A01 * f[1][h].y;
-                n1.x += A10 * f[0][h].x; n1.y += A10 * f[0][h].y;
-                n1.x += A11 * f[1][h].x; n1.y += A11 * f[1][h].y;
+                cx<R> nn = y;
+                R sq = (R)0;
+#pragma unroll
+                for (int i = 0; i < S; ++i) {
+                    cx<R> ni = y;
+#pragma unroll
+                    for (int k = 0; k < S; ++k) {
+                        ni.x += A[i][k] * f[k][h].x;
+                        ni.y += A[i][k] * f[k][h].y;
+                    }
+                    const R dx = yk[i][h].x - ni.x, dy = yk[i][h].y - ni.y;
+                    sq += dx * dx;
+                    sq += dy * dy;
+                    yk[i][h] = ni;
+                }
                 // ODEStepper.cpp:279-285 (final combination, in case this is the last iterate)
-                nn.x += B0 * f[0][h].x; nn.y += B0 * f[0][h].y;
-                nn.x += B1 * f[1][h].x; nn.y += B1 * f[1][h].y;
-                const R d0x = yk[0][h].x - n0.x, d0y = yk[0][h].y - n0.y;
-                const R d1x = yk[1][h].x - n1.x, d1y = yk[1][h].y - n1.y;
-                loc += d0x * d0x + d0y * d0y + d1x * d1x + d1y * d1y;
-                yk[0][h] = n0;
-                yk[1][h] = n1;
+#pragma unroll
+                for (int i = 0; i < S; ++i) {
+                    nn.x += B[i] * f[i][h].x;
+                    nn.y += B[i] * f[i][h].y;
+                }
+                loc += sq;
                 ys[1][t + T * h] = nn;
             }
             if (t == T - 2) {
                 bc[0] = yk[0][H - 1].x;
-                bc[1] = yk[1][H - 1].x;
+                bc[1] = yk[S == 1 ? 0 : 1][H - 1].x;
+                if constexpr (S == 3) bc[4] = bc[5] = yk[2][H - 1].x;
             }
             const R norm2 = block_sum<T, R>(loc, red, t);
-            __syncthreads();                                  // publishes bc[0..1]
-            const double rms = sqrt((double)norm2 / (double)(N * 2));
+            __syncthreads();                                  // publishes the fix values in bc
+            const double rms = sqrt((double)norm2 / (double)(N * S));
             double tol = a.precision;
             if (its > a.maxIts / 2) tol = a.precision * 10.0 * (2.0 * (double)its / (double)a.maxIts - 0.5);
             if (rms < tol) {

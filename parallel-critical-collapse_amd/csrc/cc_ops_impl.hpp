@@ -41,6 +41,23 @@ hipError_t Ops<N>::chain(hipStream_t s, int T, int nchains, const ChainParams<do
     constexpr int DB = N <= 1024 ? 1 : 0;
     const bool dbl = nchains <= kFewChains;
     const dim3 g((unsigned)nchains);
+    if (a.stages != 2) {                       // SchemeIRK 1 or 3: kThreads, no record
+        if (rec_fp64 || a.rec || T != kThreads) return hipErrorInvalidValue;
+        if (a.stages == 1) {
+            if (dbl)
+                hipLaunchKernelGGL((k_chain<N, kThreads, double, float, DB, 1>), g, dim3(kThreads), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_chain<N, kThreads, double, float, 0, 1>), g, dim3(kThreads), 0, s, a);
+        } else if (a.stages == 3) {
+            if (dbl)
+                hipLaunchKernelGGL((k_chain<N, kThreads, double, float, DB, 3>), g, dim3(kThreads), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_chain<N, kThreads, double, float, 0, 3>), g, dim3(kThreads), 0, s, a);
+        } else {
+            return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (rec_fp64) {
         if (T != kThreads) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_chain<N, kThreads, double, double, DB>), g, dim3(kThreads), 0, s, a);
@@ -66,7 +83,7 @@ hipError_t Ops<N>::chain(hipStream_t s, int T, int nchains, const ChainParams<do
 template <int N>
 hipError_t Ops<N>::chain(hipStream_t s, int T, int nchains, const ChainParams<float> &a)
 {
-    if (T != kThreads) return hipErrorInvalidValue;
+    if (T != kThreads || a.stages != 2) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_chain<N, kThreads, float>), dim3((unsigned)nchains), dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }

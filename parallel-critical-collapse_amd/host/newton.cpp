@@ -105,8 +105,13 @@ NewtonSolver::NewtonSolver(const SimulationConfig &config, const SolverOptions &
 {
     N_ = cfg_.Ntau;
     n_ = 3 * N_ / 4;
-    if (cfg_.SchemeIRK != 2)
-        throw std::invalid_argument("the device path integrates with SchemeIRK 2 (Gauss-Legendre, 2 stages)");
+    if (cfg_.SchemeIRK != 2 && opt_.policy == Policy::Mixed) {
+        // the tangent-linear columns follow the two-stage scheme; other schemes
+        // assemble the Jacobian from FP64 finite differences, as the reference does
+        opt_.policy = Policy::FP64;
+        if (opt_.verbose)
+            std::cout << "SchemeIRK " << cfg_.SchemeIRK << ": FP64 finite-difference Jacobian" << std::endl;
+    }
     if (!cfg_.has_delta || (int)cfg_.fc.size() != N_ || (int)cfg_.psic.size() != N_ || (int)cfg_.Up.size() != N_)
         throw std::invalid_argument("Initial_Conditions must hold Delta and fc, psic, Up of length Ntau");
     cc_problem p{};

@@ -181,3 +181,22 @@ def test_cc_newton_device_loop_writes_the_same_result(anchored_config, tmp_path)
         assert r.returncode == 0, r.stderr
         outs.append(json.loads(out.read_text()))
     assert outs[0] == outs[1]
+
+
+def test_cc_newton_irk3_input(anchored_config, tmp_path):
+    """An input with SchemeIRK 3: cc_newton assembles the Jacobian from FP64
+    finite differences (the tangent-linear columns follow the two-stage
+    scheme) and converges; the result keeps the scheme."""
+    shape, _, d, _ = anchored_config
+    d3 = dict(d, SchemeIRK=3, MaxIterNewton=12)
+    p = tmp_path / "irk3.json"
+    p.write_text(json.dumps(d3))
+    out = tmp_path / "irk3_out.json"
+    print("cc_newton SchemeIRK 3", flush=True)
+    r = subprocess.run([CLI, "-i", str(p), "-o", str(out)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    assert "SchemeIRK 3: FP64 finite-difference Jacobian" in r.stdout
+    res = json.loads(out.read_text())
+    assert res["Converged"] is True and res["SchemeIRK"] == 3
+    assert res["mismatchNorm"] < d3["PrecisionNewton"]

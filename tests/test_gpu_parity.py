@@ -419,3 +419,60 @@ def test_mixed_jacobian_matches_fp64_tangent_larger_ntau(N, X, seed_off):
         assert err < tol
     assert rel(J_m[:, dslot], J_t[:, dslot]) < 1e-5
     assert rel(J_m[:, keep], J_t[:, keep]) < 2e-3
+
+
+def scheme_case(scheme):
+    """IRK1 (implicit midpoint) needs the 2000-point grid for the reference's
+    fixed point to converge from the seeded input; IRK3 runs on the 1000-point
+    shape."""
+    X = 2000 if scheme == 1 else 1000
+    shape = W.Shape(Ntau=128, X=X, seed=W.config(1).seed, SchemeIRK=scheme)
+    cfg = O.Config(Ntau=128, NLeft=shape.NLeft, NRight=shape.NRight, XMid=shape.XMid, Delta=shape.Delta,
+                   scheme=scheme)
+    return shape, cfg, W.initial_x(shape)
+
+
+@pytest.mark.parametrize("scheme", [1, 3])
+def test_other_irk_schemes_match_oracle(scheme):
+    """SchemeIRK 1 and 3 (ODEStepper.cpp:29-78) in the FP64 chains: 20 steps
+    of state and iteration counts, whole sides, F(x) and FD Jacobian columns
+    of every field against the oracle."""
+    shape, cfg, x = scheme_case(scheme)
+    o = O.Oracle(cfg)
+    ctx = cc.Context(cc.Problem.from_shape(shape))
+    N = 128
+    up, ps, fc = o.unpack(np.where(np.arange(x.size) == N // 2 + 2, 0.0, x))
+    YL = o.left_expansion(fc, ps, x[N // 2 + 2])
+    grid = o.grid()
+    assert np.array_equal(grid, ctx.grid())
+    Y, its_ref = YL.copy(), 0
+    for i in range(20):
+        Y, conv, its = o.step_irk(Y, grid[i], grid[i + 1])
+        assert conv
+        its_ref += its
+    print("scheme", scheme, "20 steps", flush=True)
+    got, its, st = ctx.integrate(YL[None, :], grid, [0], [20])
+    assert st[0] == 0
+    assert rel(got[0], Y) < 1e-12
+    assert abs(int(its[0]) - its_ref) <= 2
+    F_ref, st, (_, _, YLm, _) = o.shoot_states(x)
+    assert st == 0
+    F = ctx.residual(x)
+    assert np.max(np.abs(F - F_ref)) / np.max(np.abs(YLm)) < 1e-9
+    cols = [1, 40, 70, N // 2 + 2, 90]             # Up, psic, fc, Delta, fc
+    J_ref, cst, _, worst = O.jacobian(cfg, x, F_ref, cols=cols)
+    assert worst == 0
+    J, cs, st = ctx.jacobian(x, F)
+    assert st == 0 and not cs.any()
+    for c in cols:
+        # FD columns carry ~1e-7 noise per entry (rounding of F / eps = 1e-8)
+        err = np.linalg.norm(J[:, c] - J_ref[:, c])
+        assert err < 1e-5 * np.linalg.norm(J_ref[:, c]) + 1e-6, (c, err)
+
+
+def test_other_irk_schemes_need_fp64_policy():
+    """The tangent-linear columns follow the two-stage scheme: a mixed context
+    with SchemeIRK 3 is rejected."""
+    shape, _, _ = scheme_case(3)
+    with pytest.raises(cc.CCError):
+        cc.Context(cc.Problem.from_shape(shape, precision=cc.CC_MIXED))

@@ -258,6 +258,44 @@ def test_irk_step_is_fourth_order():
     assert 14 < e1 / e2 < 18 and 14 < e2 / e3 < 18
 
 
+def test_irk_midpoint_is_second_order():
+    """Gauss-Legendre s=1 (implicit midpoint, ODEStepper.cpp:29-39) is 2nd
+    order: halving the step shrinks the error of a fixed x-interval by ~4."""
+    s = shape_c1(128, 100)
+    s["cfg"].scheme = 1
+    o = oracle_for(s)
+    x = s["x"]
+    N = 128
+    up, ps, fc = o.unpack(np.where(np.arange(x.size) == N // 2 + 2, 0.0, x))
+    Y0 = o.left_expansion(fc, ps, x[N // 2 + 2])
+    finals = []
+    for nsteps in (8, 16, 32, 64):
+        g = np.linspace(0.001, 0.004, nsteps + 1)
+        Y, st, _ = o.integrate(Y0, g, 0, nsteps)
+        assert st == 0
+        finals.append(Y)
+    errs = [np.linalg.norm(finals[i] - finals[i + 1]) for i in range(3)]
+    assert 3.5 < errs[0] / errs[1] < 4.5 and 3.5 < errs[1] / errs[2] < 4.5
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3])
+def test_gauss_legendre_tableaus(scheme):
+    """The tableaus (ODEStepper.cpp:29-78) are the Gauss-Legendre ones, which
+    the order conditions fix uniquely: the nodes are the roots of the shifted
+    Legendre polynomial P_s(2c-1), sum_i b_i c_i^(k-1) = 1/k for k <= 2s
+    (B(2s)) and sum_j a_ij c_j^(k-1) = c_i^k / k for k <= s (C(s))."""
+    s = shape_c1(128, 100)
+    s["cfg"].scheme = scheme
+    a, b, c = oracle_for(s).tableau()
+    assert a.shape == (scheme, scheme)
+    roots = np.sort((np.polynomial.legendre.legroots([0] * scheme + [1]) + 1) / 2)
+    assert np.max(np.abs(np.sort(c) - roots)) < 1e-15
+    for k in range(1, 2 * scheme + 1):
+        assert abs(np.sum(b * c ** (k - 1)) - 1 / k) < 1e-15, k
+    for k in range(1, scheme + 1):
+        assert np.max(np.abs(a @ c ** (k - 1) - c ** k / k)) < 1e-15, k
+
+
 def test_irk_nonconvergence_is_reported():
     """MaxIterIRK=1 cannot meet PrecisionIRK=1e-15 (ShootingSolver.cpp:69-82)."""
     s = shape_c1(128, 100)
