@@ -85,9 +85,21 @@ struct RhsConst {
     R dm2, dm3, c3, two_dm3, mg, q2;  // D-2, D-3, (D-2)^3, 2(D-3), -(D-3), (k0 N/2)^2
 };
 
+// DB = 1 evaluations run the two stages' first inverse transforms as one
+// batch of four signals (one set of exchanges and barriers instead of two; the
+// butterflies of each signal are unchanged, so are the results).  Their LDS
+// regions then hold 4N complex; rhs_lds gives the exchange buffer a caller
+// declares, (1 + DB) regions.
+template <int N, typename R, int DB>
+constexpr bool rhs_merge() { return DB != 0; }
+template <int N, typename R, int DB>
+constexpr int rhs_region() { return (rhs_merge<N, R, DB>() ? 4 : 2) * N; }
+template <int N, typename R, int DB>
+constexpr int rhs_lds() { return (1 + DB) * rhs_region<N, R, DB>(); }
+
 // Evaluate f_s = RHS(yk_s, xs_s) for both stages (ODEStepper.cpp:92-136).
 // bc[0], bc[1]: Re of mode M-2 of each stage state (the mode-2 fix value),
-// published by the caller before the call.
+// published by the caller before the call.  xb: rhs_lds<N, R, DB>() complex.
 template <int N, int T, typename R, typename RT = float, int DB = 0>
 __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> (&f)[2][N / T / 2],
                                          const R (&xs)[2], const RhsConst<R> &K, const R *__restrict__ invm,
@@ -106,8 +118,10 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     // (fft_next_par); the spectrum separation takes the next region and the
     // transform after it the other one.  The caller's reduction barrier
     // separates the last transform from the next evaluation.
-    constexpr int RS = 2 * N;
-    constexpr int pA = 0, pB = DB ? fft_next_par<N, T>(pA) : 0, pC = DB ? fft_next_par<N, T>(pB) : 0;
+    constexpr bool MERGE = rhs_merge<N, R, DB>();
+    constexpr int RS = rhs_region<N, R, DB>();
+    constexpr int pA = 0, pB = DB ? fft_next_par<N, T>(pA) : 0;
+    constexpr int pC = DB ? fft_next_par<N, T>(MERGE ? pA : pB) : 0;
     constexpr int pD = DB ? fft_next_par<N, T>(pC) : 0, pE = DB ? fft_next_par<N, T>(pD) : 0;
     constexpr int pS = DB ? fft_next_par<N, T>(pE) : 0, pG = DB ? 1 - pS : 0;
     constexpr int pH = DB ? fft_next_par<N, T>(pG) : 0;
@@ -115,41 +129,8 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     // per-stage products of the split that survive until the pointwise RHS
     R AU[2][H], BU[2][H], AV[2][H], BV[2][H], CF[2][H];
     cx<R> c[P];                                   // packed Coeff1 of both stages
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        // (1) doubled spectrum + mode-2 fix (ICG.cpp:797-800), tau-derivative (803)
-        cx<R> z[2][P];
-        const R fix = bc[s];
-#pragma unroll
-        for (int h = 0; h < H; ++h) {
-            const int q = t + T * h;
-            const cx<R> Y = yk[s][h];
-            cx<R> lo, hi;
-            if (q < M / 2) {
-                lo = Y;
-                hi = {(R)0, (R)0};
-            } else if (q == M / 2) {
-                lo = cscale(Y, half);
-                hi = Y;
-            } else {
-                lo = {(R)0, (R)0};
-                hi = Y;
-            }
-            if (q == 2) lo.x = -fix;
-            z[0][h] = lo;
-            z[0][h + H] = hi;
-        }
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const int k = t + T * p;
-            const R w = (k == N / 2) ? (R)0 : (R)wavenumber<N>(k) * K.k0;
-            const cx<R> v = z[0][p];
-            z[1][p] = {w * v.y, -(w * v.x)};
-        }
-        // (2) back to tau (804-805)
-        if (s == 0) fft_block<N, T, -1, 2, R, DB, pA, 0, RS>(z, xb, tw, t);
-        else fft_block<N, T, -1, 2, R, DB, pB, 0, RS>(z, xb, tw, t);
-        // (3) half-period split (814-827), IA2 coefficients (834-836), RHS products
+    // (3) half-period split (814-827), IA2 coefficients (834-836), RHS products
+    auto split = [&](int s, const cx<R> (&z)[2][P]) {
         const R X = xs[s];
         const R inv_x = (R)1 / X;
 #pragma unroll
@@ -179,6 +160,50 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
             BV[s][h] = K.two_dm3 * F * V * inv2;
             CF[s][h] = K.dm3 * F * inv_x;
         }
+    };
+    cx<R> zz[2][2][P];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        // (1) doubled spectrum + mode-2 fix (ICG.cpp:797-800), tau-derivative (803)
+        cx<R>(&z)[2][P] = zz[s];
+        const R fix = bc[s];
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            const int q = t + T * h;
+            const cx<R> Y = yk[s][h];
+            cx<R> lo, hi;
+            if (q < M / 2) {
+                lo = Y;
+                hi = {(R)0, (R)0};
+            } else if (q == M / 2) {
+                lo = cscale(Y, half);
+                hi = Y;
+            } else {
+                lo = {(R)0, (R)0};
+                hi = Y;
+            }
+            if (q == 2) lo.x = -fix;
+            z[0][h] = lo;
+            z[0][h + H] = hi;
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int k = t + T * p;
+            const R w = (k == N / 2) ? (R)0 : (R)wavenumber<N>(k) * K.k0;
+            const cx<R> v = z[0][p];
+            z[1][p] = {w * v.y, -(w * v.x)};
+        }
+        // (2) back to tau (804-805)
+        if constexpr (!MERGE) {
+            if (s == 0) fft_block<N, T, -1, 2, R, DB, pA, 0, RS>(z, xb, tw, t);
+            else fft_block<N, T, -1, 2, R, DB, pB, 0, RS>(z, xb, tw, t);
+            split(s, z);
+        }
+    }
+    if constexpr (MERGE) {
+        fft_block<N, T, -1, 4, R, DB, pA, 0, RS>(*reinterpret_cast<cx<R>(*)[4][P]>(&zz), xb, tw, t);
+        split(0, zz[0]);
+        split(1, zz[1]);
     }
 #pragma unroll
     for (int h = 0; h < H; ++h) c[h + H] = c[h];
@@ -325,7 +350,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
 {
     static_assert(S >= 1 && S <= 3, "Gauss-Legendre schemes with 1, 2 or 3 stages");
     constexpr int P = N / T, H = P / 2, M = N / 2;
-    __shared__ cx<R> xb[(1 + DB) * 2 * N];
+    __shared__ cx<R> xb[rhs_lds<N, R, DB>()];
     __shared__ R bc[8];
     __shared__ R red[(T / 64) > 0 ? T / 64 : 1];
     const int t = threadIdx.x;
@@ -474,7 +499,7 @@ __global__ __launch_bounds__(T) void k_rhs_pairs(const cx<R> *Y, const double *x
                                                  cx<R> *dY, const cx<R> *tw, const R *invm, double Dim, R *rec)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
-    __shared__ cx<R> xb[2 * N];
+    __shared__ cx<R> xb[rhs_lds<N, R, 0>()];
     __shared__ R bc[8];
     const int t = threadIdx.x;
     const int pr = blockIdx.x;
