@@ -305,18 +305,32 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
         const R B0 = (R)(dxd * a.b0), B1 = (R)(dxd * a.b1);
         const R *rec = a.rec + (size_t)(a.rec_base[ch] + step) * Rec<N>::STEP;
         // Initial stage iterates: the step-start state, or (after a chain's
-        // first step) the stages the previous step's derivatives predict,
-        // K_s = y + dx sum_j a_sj f_j(prev).  The linear fixed point then starts
-        // O(dx^2) instead of O(dx) from its solution (the reference starts
-        // from K_s = y, ODEStepper.cpp:193-199).
+        // first step) the previous step's collocation polynomial extrapolated
+        // to this step's stage points: with u' = sum_j f_j(prev) L_j(theta)
+        // (L_j the Lagrange basis on the nodes c, theta in units of the
+        // previous step hp), K_s = y + hp sum_j f_j int_1^{1 + r c_s} L_j, r =
+        // dx / hp, i.e. E_sj = dx c_s (2 (1 - c_o) + r c_s) / (2 (c_j - c_o))
+        // with o the other node.  The linear fixed point then starts O(dx^3)
+        // from its solution, against O(dx) from K_s = y (the reference's
+        // start, ODEStepper.cpp:193-199).  A start value: computed in R.
+        R E00 = R(0.0), E01 = R(0.0), E10 = R(0.0), E11 = R(0.0);
+        if (PRED && have_prev) {
+            const R r = (R)dxd * trcp((R)(xin - grid[idx - dir]));
+            const R c0 = (R)a.c0, c1 = (R)a.c1, g = R(0.5) * (R)dxd * trcp(c0 - c1);
+            const R u0 = R(2.0) * (R(1.0) - c0), u1 = R(2.0) * (R(1.0) - c1);
+            E00 = g * c0 * (u1 + r * c0);
+            E01 = -(g * c0 * (u0 + r * c0));
+            E10 = g * c1 * (u1 + r * c1);
+            E11 = -(g * c1 * (u0 + r * c1));
+        }
 #pragma unroll
         for (int h = 0; h < H; ++h) {
             const cx<R> y = ys[0][t + T * h];
             dk[0][h] = dk[1][h] = y;
             if (PRED && have_prev) {
                 const cx<R> f0 = fs[0][t + T * h], f1 = fs[1][t + T * h];
-                dk[0][h] = {y.x + A00 * f0.x + A01 * f1.x, y.y + A00 * f0.y + A01 * f1.y};
-                dk[1][h] = {y.x + A10 * f0.x + A11 * f1.x, y.y + A10 * f0.y + A11 * f1.y};
+                dk[0][h] = {y.x + E00 * f0.x + E01 * f1.x, y.y + E00 * f0.y + E01 * f1.y};
+                dk[1][h] = {y.x + E10 * f0.x + E11 * f1.x, y.y + E10 * f0.y + E11 * f1.y};
             }
         }
         if (t == T - 2) {
