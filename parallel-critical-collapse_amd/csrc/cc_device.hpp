@@ -114,23 +114,57 @@ __device__ __forceinline__ void dft(cx<R>* v)
 // remaining factor.  All but the last pass exchange through LDS (B*N
 // complex); the last pass's outputs land back in layout L.
 // tw[e] = (cos, sin)(2 pi e / N), e in [0, N).
-// Exchange-buffer position of element i.  Stockham writes of the early
-// passes put consecutive lanes 4 (or 16) elements apart, so the lanes that
-// one LDS cycle serves (16 lanes of 8-byte FP32 complex, 8 lanes of 16-byte
-// FP64 complex: 128 bytes over 32 banks) would share a few bank slots.  FP32:
-// both low bit pairs XOR bits 4-5 of i; FP64: the low three bits XOR a map
-// of bits 3-4.  Either spreads every pass's writes and the layout-L reads
-// over all slots, and permutes each aligned block of 16 (8), so positions
-// stay in [0, N).  The map is linear over GF(2): for a and b with disjoint
-// bits (a + b == a ^ b), xswz(a + b) == xswz(a) ^ xswz(b), so a position
-// made of a per-thread part and a compile-time offset costs one XOR.
-template <int N, typename R>
+// Exchange-buffer position of element i in the exchange after a pass with
+// sub-transform length NS and radix RAD.  Stockham writes of the early passes
+// put consecutive lanes RAD elements apart, so the lanes that one LDS cycle
+// serves (16 lanes of 8-byte FP32 complex, 8 lanes of 16-byte FP64 complex:
+// 128 bytes over 32 banks) would share a few bank slots.  FP32: both low bit
+// pairs XOR bits 4-5 of i, in every pass.  FP64: per pass, each of the lane
+// bits 0-2 that the write sends above position bit 2 (to bit m + log2 RAD
+// when 2^m >= NS) is XORed into a low position bit no lane bit occupies, and
+// nothing else: 8 consecutive lanes then write 8 distinct slots, and passes
+// whose lane bits stay low (NS >= 8) keep the identity.  Measured on gfx950:
+// XORing bit 4 into bit 2 for all passes, the earlier FP64 map, made every
+// layout-L read of a 16-byte element cost 4 extra LDS cycles, which the
+// per-pass maps confine to the NS = 4 exchange of radix 4.  Each map permutes
+// aligned blocks of 8 (FP64) or 64 (FP32) positions, so positions stay in
+// [0, N), and is linear over GF(2): for a and b with disjoint bits (a + b ==
+// a ^ b), xswz(a + b) == xswz(a) ^ xswz(b), so a position made of a
+// per-thread part and a compile-time offset costs one XOR.
+template <int NS, int RAD>
+constexpr int xswz64_pairs()
+{
+    // packs up to three (source bit, destination bit) pairs, 4 bits each
+    int lg = 0;
+    while ((1 << lg) < RAD) ++lg;
+    bool low_used[3] = {false, false, false};
+    int src[3], ns = 0;
+    for (int m = 0; m < 3; ++m) {
+        const int pos = ((1 << m) < NS) ? m : m + lg;
+        if (pos < 3) low_used[pos] = true;
+        else src[ns++] = pos;
+    }
+    int packed = 0, k = 0;
+    for (int f = 0; f < 3 && k < ns; ++f)
+        if (!low_used[f]) {
+            packed |= ((src[k] << 4) | f) << (8 * k);
+            ++k;
+        }
+    return packed | (k << 24);
+}
+
+template <int N, typename R, int NS, int RAD>
 __device__ __forceinline__ int xswz(int i)
 {
     if constexpr (N < 64) {
         return i;
     } else if constexpr (sizeof(R) == 8) {
-        return i ^ (((i >> 3) & 1) ^ (((i >> 4) & 1) * 6));
+        constexpr int pk = xswz64_pairs<NS, RAD>(), np = pk >> 24;
+        int x = i;
+        if constexpr (np > 0) x ^= ((i >> ((pk >> 4) & 15)) & 1) << (pk & 15);
+        if constexpr (np > 1) x ^= ((i >> ((pk >> 12) & 15)) & 1) << ((pk >> 8) & 15);
+        if constexpr (np > 2) x ^= ((i >> ((pk >> 20) & 15)) & 1) << ((pk >> 16) & 15);
+        return x;
     } else {
         const int g = (i >> 4) & 3;
         return i ^ (g | (g << 2));
@@ -195,9 +229,9 @@ __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const
                 for (int r = 0; r < RAD; ++r) v[s][b + r * NB] = u[r];
             } else {
                 // base has no bits in common with r * NS (base mod NS*RAD == k < NS)
-                const int sb = xswz<N, R>((j / NS) * NS * RAD + k);
+                const int sb = xswz<N, R, NS, RAD>((j / NS) * NS * RAD + k);
 #pragma unroll
-                for (int r = 0; r < RAD; ++r) lds[PAR * RS + s * N + (sb ^ xswz<N, R>(r * NS))] = u[r];
+                for (int r = 0; r < RAD; ++r) lds[PAR * RS + s * N + (sb ^ xswz<N, R, NS, RAD>(r * NS))] = u[r];
             }
         }
     }
@@ -206,7 +240,7 @@ __device__ __forceinline__ void fft_pass(cx<R> (&v)[B][N / T], cx<R>* lds, const
 #pragma unroll
         for (int s = 0; s < B; ++s)
 #pragma unroll
-            for (int p = 0; p < P; ++p) v[s][p] = lds[PAR * RS + s * N + (xswz<N, R>(t) ^ xswz<N, R>(T * p))];
+            for (int p = 0; p < P; ++p) v[s][p] = lds[PAR * RS + s * N + (xswz<N, R, NS, RAD>(t) ^ xswz<N, R, NS, RAD>(T * p))];
         if constexpr (!DB) __syncthreads();
         fft_pass<N, T, SIGN, B, NS * RAD, R, DB, DB ? 1 - PAR : 0, RS>(v, lds, tw, t);
     }
