@@ -108,10 +108,11 @@ def time_policy(cc, dev, shape, precision, steps, warmup, reuse, tune):
     }
 
 
-def cpu_baseline(shape, threads, budget_s):
+def cpu_baseline(shape, threads, ncols):
     """The oracle (CPU restatement, OpenMP over Jacobian columns like the
-    reference's cc_openmp build) on a bounded sample: one column per thread,
-    spread over the Up, psic, fc slots and Delta; scaled to Newton steps/s."""
+    reference's cc_openmp build) on a bounded sample of ncols columns (rounded
+    up to a multiple of the thread count), spread over the Up, psic, fc slots
+    and Delta; scaled to Newton steps/s by (n+1)/ncols."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
@@ -121,7 +122,8 @@ def cpu_baseline(shape, threads, budget_s):
                    EpsNewton=shape.EpsNewton, PrecisionIRK=shape.PrecisionIRK, MaxIterIRK=shape.MaxIterIRK)
     W = importlib.import_module("parallel-critical-collapse_amd.workload")
     x = W.initial_x(shape)
-    cols = sorted(set(int(round(v)) for v in np.linspace(0, n - 1, threads)) | {shape.Ntau // 2 + 2})[:threads]
+    k = -(-max(ncols, threads) // threads) * threads
+    cols = sorted(set(int(round(v)) for v in np.linspace(0, n - 1, k)) | {shape.Ntau // 2 + 2})[:k]
     F0 = np.zeros(n)
     t = time.perf_counter()
     J, status, cnt, worst = O.jacobian(cfg, x, F0, cols=cols, nthreads=threads)
@@ -129,8 +131,9 @@ def cpu_baseline(shape, threads, budget_s):
     cols_per_s = len(cols) / dt
     return {
         "value": cols_per_s / (n + 1), "unit": "Newton steps/s", "cores": threads, "kind": "port",
-        "sample": f"{len(cols)} FD Jacobian columns (full shoots, Ntau={shape.Ntau}, {shape.X} x-points) "
-                  f"in {dt:.1f} s on {threads} threads; steps/s = columns/s / (n+1), n={n}",
+        "sample": f"{len(cols)} FD Jacobian columns spread over Up, psic, fc and Delta (full shoots, "
+                  f"Ntau={shape.Ntau}, {shape.X} x-points) in {dt:.1f} s on {threads} threads; "
+                  f"extrapolated: steps/s = columns/s / (n+1), n={n}",
         "status_ok": bool(worst == 0), "kbar": float(cnt[1] / max(cnt[0], 1)),
     }
 
@@ -287,6 +290,8 @@ def main():
     ap.add_argument("--schedule", default="", help='fixed schedule fields as JSON, e.g. \'{"threads_tangent": 128}\'; '
                     "skips the autotuner")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-columns", type=int, default=64,
+                    help="Jacobian columns of the CPU baseline sample (rounded up to a multiple of the threads)")
     ap.add_argument("--newton", action="store_true",
                     help="Newton to convergence under each of --policies instead of the step benchmark")
     ap.add_argument("--policies", default="mixed,fp64_tangent,fp64")
@@ -370,7 +375,7 @@ def main():
         out[f"mixed_over_{key}"] = round(mixed["steps_per_s"] / res["steps_per_s"], 3)
     if not args.no_cpu:
         log(f"[bench] cpu baseline on {args.cpu_threads} threads")
-        out["cpu_baseline"] = cpu_baseline(shape, args.cpu_threads, 30.0)
+        out["cpu_baseline"] = cpu_baseline(shape, args.cpu_threads, args.cpu_columns)
     print(json.dumps(out), flush=True)
 
 

@@ -138,7 +138,7 @@ constexpr int xswz64_pairs()
     int lg = 0;
     while ((1 << lg) < RAD) ++lg;
     bool low_used[3] = {false, false, false};
-    int src[3], ns = 0;
+    int src[3] = {0, 0, 0}, ns = 0;
     for (int m = 0; m < 3; ++m) {
         const int pos = ((1 << m) < NS) ? m : m + lg;
         if (pos < 3) low_used[pos] = true;
