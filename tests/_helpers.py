@@ -21,14 +21,17 @@ def golden(name):
         return json.load(f)
 
 
-def shape_c1(N=128, X=1000):
-    """The CPU-parity shape (1000 x-points, XMid=0.5) at Ntau=N with the seeded input."""
-    base = W.config(1)
-    shape = W.Shape(Ntau=N, X=X, seed=base.seed)
-    cfg = O.Config(Ntau=N, NLeft=shape.NLeft, NRight=shape.NRight, XMid=shape.XMid, XLeft=shape.XLeft,
+def shape_of(shape):
+    """Test case of a workload shape: the shape, its oracle configuration and seeded input."""
+    cfg = O.Config(Ntau=shape.Ntau, NLeft=shape.NLeft, NRight=shape.NRight, XMid=shape.XMid, XLeft=shape.XLeft,
                    XRight=shape.XRight, Dim=shape.Dim, Delta=shape.Delta, EpsNewton=shape.EpsNewton,
                    PrecisionIRK=shape.PrecisionIRK, MaxIterIRK=shape.MaxIterIRK)
     return {"shape": shape, "cfg": cfg, "x": W.initial_x(shape)}
+
+
+def shape_c1(N=128, X=1000):
+    """The CPU-parity shape (1000 x-points, XMid=0.5) at Ntau=N with the seeded input."""
+    return shape_of(W.Shape(Ntau=N, X=X, seed=W.config(1).seed))
 
 
 def oracle_for(s):
