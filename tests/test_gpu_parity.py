@@ -415,14 +415,14 @@ def test_fp64_tangent_newton_step(c1_reference):
     assert np.linalg.norm(J @ dx + F0) / np.linalg.norm(F0) < 1e-9
 
 
-@pytest.mark.parametrize("precision", [cc.CC_MIXED, cc.CC_FP64_TANGENT])
-def test_jacobian_directional_derivative_at_bench_shape(precision):
+@pytest.mark.parametrize("config,precision", [(3, cc.CC_MIXED), (3, cc.CC_FP64_TANGENT), (5, cc.CC_MIXED)])
+def test_jacobian_directional_derivative_at_bench_shape(config, precision):
     """J v against the central difference of the FP64 residual along a seeded
     unit direction v, at the benchmark workload's shape (Ntau=1024, 8000
-    x-points): the product the Newton step relies on, at its real size.
-    h = 1e-6 keeps the difference's truncation (h^2) and rounding (eps/h)
-    errors below 1e-9 relative."""
-    s = shape_of(W.config(3))
+    x-points) and at Ntau=2048 with 12000 x-points: the product the Newton step
+    relies on, at its real size.  h = 1e-6 keeps the difference's truncation
+    (h^2) and rounding (eps/h) errors below 1e-9 relative."""
+    s = shape_of(W.config(config))
     ctx = ctx_for(s, precision=precision)
     x = s["x"]
     F0 = ctx.residual(x)
@@ -433,9 +433,10 @@ def test_jacobian_directional_derivative_at_bench_shape(precision):
     h = 1e-6
     dF = (ctx.residual(x + h * v) - ctx.residual(x - h * v)) / (2 * h)
     err = np.linalg.norm(J @ v - dF) / np.linalg.norm(dF)
-    print(f"bench shape, precision {precision}: |J v - dF| / |dF| = {err:.2e}")
-    # measured on an MI355X: mixed 2.6e-5 (FP32 tangent columns), FP64 tangent
-    # 1.1e-6 (the Delta column's forward difference at EpsNewton = 1e-8)
+    print(f"shape {config}, precision {precision}: |J v - dF| / |dF| = {err:.2e}")
+    # measured on an MI355X: mixed 2.6e-5 (Ntau=1024) and 3.7e-5 (Ntau=2048,
+    # FP32 tangent columns), FP64 tangent 1.1e-6 (the Delta column's forward
+    # difference at EpsNewton = 1e-8)
     assert err < (3e-4 if precision == cc.CC_MIXED else 1e-5)
 
 
