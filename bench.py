@@ -371,7 +371,8 @@ def main():
         out[key] = {"value": round(res["steps_per_s"], 5), "ms_per_step": round(res["ms_per_step"], 2),
                     "kbar": round(res["kbar"], 3), "kernel": f"k_chain<{N},{res['schedule']['threads_bulk']},double>",
                     "achieved_tflops": round(ach, 3), "peak": 78.6, "frac": round(ach / 78.6, 4),
-                    "schedule": res["schedule"]}
+                    "avg_launch_ms": res["chain_ms"] / max(res["launches"], 1),
+                    "launches_per_step": res["launches"] / args.steps, "schedule": res["schedule"]}
         out[f"mixed_over_{key}"] = round(mixed["steps_per_s"] / res["steps_per_s"], 3)
     if not args.no_cpu:
         log(f"[bench] cpu baseline on {args.cpu_threads} threads")
