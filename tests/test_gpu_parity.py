@@ -250,10 +250,11 @@ def test_solve_golden_10x10():
     assert np.max(np.abs(x - ref)) < 1e-12
 
 
-@pytest.mark.parametrize("n", [7, 64, 96, 200, 768, 1100])
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 96, 200, 768, 1100, 1536])
 def test_solve_matches_oracle_dgesv(n):
-    """n = 1100 factors its first panels (more than 1024 rows) in global
-    memory, the rest in LDS."""
+    """n = 1100 and 1536 (the largest Jacobian, Ntau = 2048) factor their
+    first panels (more than 1024 rows) in global memory, the rest in LDS;
+    n = 1 and 2 are the smallest systems."""
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n)) + np.eye(n) * 0.1
     b = rng.standard_normal(n)
@@ -263,6 +264,19 @@ def test_solve_matches_oracle_dgesv(n):
     assert st == 0 and info == 0 and info_ref == 0
     assert rel(x, ref) < 1e-10
     assert rel(LU, LU_ref) < 1e-10       # same pivot sequence -> same factors
+
+
+def test_solve_rejects_sizes_beyond_the_context():
+    """cc_solve works in the context's Jacobian buffers: n above the context's
+    n, or below 1, is CC_BADARG before anything is read or launched (so the
+    one-element buffers passed here are never touched)."""
+    import ctypes as C
+    ctx = cc.Context(cc.Problem.make(Ntau=128, NLeft=10, NRight=10))
+    A, b, info = np.zeros(1), np.zeros(1), C.c_int32(0)
+    for n in (ctx.n + 1, 0, -3):
+        st = cc.load().cc_solve(ctx.h, n, A.ctypes.data_as(C.POINTER(C.c_double)),
+                                b.ctypes.data_as(C.POINTER(C.c_double)), C.byref(info))
+        assert st == cc.CC_BADARG, n
 
 
 def test_solve_singular_reports_info():
