@@ -93,6 +93,21 @@ def test_cc_newton_already_converged(anchored_config, tmp_path):
     assert res["mismatchNorm"] == rms(F)
 
 
+def test_cc_newton_reports_nonconvergence(anchored_config, tmp_path):
+    """MaxIterNewton = 1 with a PrecisionNewton one step cannot reach: the run
+    fails like the reference's (NewtonSolver.cpp:368-372): non-zero exit,
+    the reference's message, no result file."""
+    shape, p, d, x0 = anchored_config
+    d2 = dict(d, MaxIterNewton=1, PrecisionNewton=1e-14)
+    p2 = tmp_path / "fail.json"
+    p2.write_text(json.dumps(d2))
+    out = tmp_path / "result.json"
+    r = subprocess.run([CLI, "-i", str(p2), "-o", str(out), "--quiet"], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "Newton method did not converge in 1 iterations" in r.stderr + r.stdout
+    assert not out.exists()
+
+
 def test_cc_newton_debug_writes_jacobians(anchored_config, tmp_path):
     """Debug runs write jacobian_<Dim>_<i>.csv per Newton iteration
     (NewtonSolver.cpp:337-341, OutputWriter.cpp:57-74): the first one is the
