@@ -1245,6 +1245,10 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
         fprintf(stderr, "cc_ctx_create: %s: %s\n", w, hipGetErrorString(e));
         for (void *q : c->allocs) (void)hipFree(q);
         if (c->hblock) (void)hipHostFree(c->hblock);
+        for (hipEvent_t ev : {c->ev_fork, c->ev_join, c->tk0, c->tk1})
+            if (ev) (void)hipEventDestroy(ev);
+        for (hipStream_t st : {c->s0, c->s1})
+            if (st) (void)hipStreamDestroy(st);
         delete c;
         return true;
     };
