@@ -4,8 +4,8 @@ Tolerances (FP64 path vs the FP64 oracle; both compute the same arithmetic
 with different FFT rounding):
   transforms, series, RHS:      1e-13 normwise relative
   one IRK step / a whole side:  1e-11 normwise relative
-  residual F(x):                1e-9 relative to max|Y| at the match point
-                                (1e-12 at the workload shapes Ntau = 512, 1024, 2048)
+  residual F(x):                1e-12 relative to max|Y| at the match point
+                                (1e-9 for the one- and three-stage schemes)
   Jacobian (FD, h = 1e-8):      1e-5 Frobenius relative
   Newton step dx:               1e-4 normwise relative
 FP32 unit transforms: 2e-6 normwise relative.
@@ -154,7 +154,9 @@ def test_residual_matches_oracle_c1():
     assert st == 0
     F = cc_residual(s)
     scale = np.max(np.abs(YLm))
-    assert np.max(np.abs(F - F_ref)) / scale < 1e-9
+    # the 1e-12 relative bound the 1000-point configuration states (measured
+    # on an MI355X: 5.1e-16 absolute, |Y| ~ 1)
+    assert np.max(np.abs(F - F_ref)) / scale < 1e-12
 
 
 def cc_residual(s):
@@ -230,7 +232,7 @@ def test_newton_step_matches_oracle_c1(c1_reference):
     s, F0_ref, J_ref, dx_ref, _ = c1_reference
     N = s["shape"].Ntau
     F0, dx = ctx_for(s).newton_step(s["x"])
-    assert np.max(np.abs(F0 - F0_ref)) < 1e-9
+    assert np.max(np.abs(F0 - F0_ref)) < 1e-12
     up, fc = slice(0, N // 4), slice(N // 2, 3 * N // 4)
     assert rel(dx[up], dx_ref[up]) < 1e-2
     assert rel(dx[fc], dx_ref[fc]) < 2e-3
@@ -343,7 +345,7 @@ def test_mixed_newton_step_solves_fp64_refined_system(c1_reference):
     s, F0_ref, _, _, _ = c1_reference
     ctx = ctx_for(s, precision=cc.CC_MIXED)
     F0, dx = ctx.newton_step(s["x"])
-    assert np.max(np.abs(F0 - F0_ref)) < 1e-9
+    assert np.max(np.abs(F0 - F0_ref)) < 1e-12
     J, _, _ = ctx.jacobian(s["x"], F0)
     assert np.linalg.norm(J @ dx + F0) / np.linalg.norm(F0) < 1e-8
 
@@ -424,7 +426,7 @@ def test_fp64_tangent_newton_step(c1_reference):
     s, F0_ref, _, _, _ = c1_reference
     ctx = ctx_for(s, precision=cc.CC_FP64_TANGENT)
     F0, dx = ctx.newton_step(s["x"])
-    assert np.max(np.abs(F0 - F0_ref)) < 1e-9
+    assert np.max(np.abs(F0 - F0_ref)) < 1e-12
     J, _, _ = ctx.jacobian(s["x"], F0)
     assert np.linalg.norm(J @ dx + F0) / np.linalg.norm(F0) < 1e-9
 

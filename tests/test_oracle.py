@@ -386,3 +386,56 @@ def test_dgesv_singular_info():
     A = np.ones((4, 4))
     _, info, _, _ = O.dgesv(A, np.ones(4))
     assert info == 2
+
+
+# ------------------------------------------------------------ grid and norm
+@pytest.mark.parametrize("NL,NR,XL,XM,XR", [(500, 500, 1e-3, 0.5, 0.999), (4000, 4000, 1e-3, 0.5, 0.999),
+                                             (3000, 3000, 1e-3, 0.1, 0.999), (7, 11, 0.02, 0.3, 0.95)])
+def test_grid_is_uniform_in_logit(NL, NR, XL, XM, XR):
+    """generateGrid (NewtonSolver.cpp:477-519) against what fixes the grid:
+    the three anchors are exact, logit(x_j) = ln x_j - ln(1 - x_j) is evenly
+    spaced on each block with NLeft resp. NRight segments, the grid is strictly
+    increasing, and every point equals the logistic function 1 / (1 + e^{-z_j})
+    of its logit node evaluated in 40-digit decimal arithmetic (a different
+    formula, in a different precision) to 1e-14 relative: the double node
+    z_j = z_L + j dz carries ~|z| eps of rounding, which moves x_j by a few
+    ulps, while a wrong node index moves it by ~1e-2."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 40
+    o = O.Oracle(O.Config(Ntau=128, NLeft=NL, NRight=NR, XLeft=XL, XMid=XM, XRight=XR))
+    g = o.grid()
+    assert g.size == NL + NR + 1
+    assert g[0] == XL and g[NL] == XM and g[NL + NR] == XR
+    assert np.all(np.diff(g) > 0)
+    z = np.log(g) - np.log1p(-g)
+    for lo, hi, m in ((0, NL, NL), (NL, NL + NR, NR)):
+        dz = np.diff(z[lo:hi + 1])
+        assert np.max(np.abs(dz - (z[hi] - z[lo]) / m)) < 1e-12
+
+    def logit(v):
+        v = Decimal(v)
+        return (v / (1 - v)).ln()
+
+    zl, zm, zr = logit(XL), logit(XM), logit(XR)
+    for j in list(range(1, min(NL, 40))) + list(range(max(1, NL - 40), NL)) + \
+            list(range(NL + 1, min(NL + NR, NL + 40))) + list(range(max(NL + 1, NL + NR - 40), NL + NR)):
+        zj = zl + (zm - zl) * j / NL if j < NL else zm + (zr - zm) * (j - NL) / NR
+        exact = float(1 / (1 + (-zj).exp()))
+        assert abs(g[j] - exact) <= 1e-14 * exact, j
+
+
+def test_l2norm_is_rms():
+    """computeL2Norm (NewtonSolver.cpp:763-768) is sqrt(mean(v^2)): against an
+    exactly rounded sum (math.fsum) on random vectors, and the cases the
+    definition fixes: a constant vector, a single entry, the zero vector and
+    homogeneity under scaling by a power of two."""
+    import math
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 96, 768, 1536):
+        v = rng.standard_normal(n) * 10.0 ** rng.integers(-8, 3)
+        exact = math.sqrt(math.fsum(float(a) * float(a) for a in v) / n)
+        assert abs(O.l2norm(v) - exact) <= 1e-14 * exact
+        assert O.l2norm(8.0 * v) == 8.0 * O.l2norm(v)
+    assert O.l2norm(np.full(768, -3.25)) == 3.25
+    assert O.l2norm(np.array([-7.5])) == 7.5
+    assert O.l2norm(np.zeros(10)) == 0.0
