@@ -14,7 +14,19 @@ Prints one JSON line.  value = Newton steps/s of the mixed-precision policy
 refinement); the FP64 form (every column a forward difference of two FP64
 shoots, as the reference) is timed in the same run and reported beside it,
 with and without reuse of the unchanged side.  Each policy is autotuned
-first (outside the timed region).
+first (outside the timed region).  Beside the headline the same line carries:
+  roofline / roofline_base   the dominant kernel (the FP32 tangent chains) and
+                             the critical-path FP64 base chains
+  oracle_parity              the Jacobian columns of both policies against the
+                             oracle's (the columns the CPU baseline computes)
+                             and the Newton steps against the oracle's step
+                             (tests/golden/step_c3_oracle.json)
+  newton                     Newton to convergence under both policies: the
+                             converged Delta and its agreement
+  fp64_ntau512_x6000         the plain FP64 form at Ntau=512, 6000 x-points
+  batch                      the 24 batched uneven problems
+  cpu_baseline               the oracle on the host cores (kind "port")
+--no-extras leaves out newton, fp64_ntau512_x6000 and batch.
 """
 import argparse
 import importlib
@@ -28,6 +40,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
 # HBM traffic per launch of the dominant kernel, from two rocprofv3 --pmc
@@ -38,6 +51,9 @@ sys.path.insert(0, ROOT)
 MEASURED_TRAFFIC_BYTES = {
     ("k_tangent<1024,256>", 64): (59.59e6 + 24.53e6),
 }
+
+FP64_PEAK, FP32_PEAK, CUS = 78.6, 157.3, 256
+DF = 1e-14          # bound on the per-entry difference of two FP64 shoots (different FFT rounding)
 
 
 def log(msg):
@@ -54,13 +70,21 @@ def flops_chain(N, its, steps, tangent=False):
     return its * (2 * frhs + 18 * N) + steps * 6 * N
 
 
+def hbm_bytes_step(shape):
+    """Algorithmic HBM bytes of one Newton step: the residual rows and the
+    Jacobian written and read back, the grid and the x/F0/dx vectors."""
+    n = shape.n
+    return 8.0 * (n * (n + 1) + n * n + shape.X + 3 * n)
+
+
 def time_policy(cc, dev, shape, precision, steps, warmup, reuse, tune):
     """Autotune (outside the timed region), then time `steps` Newton steps on
-    device-resident x, F0, dx."""
+    device-resident x, F0, dx.  Also returns the Jacobian at x."""
     n = shape.n
     ctx = cc.Context(cc.Problem.from_shape(shape, precision=precision, reuse_sides=int(reuse)))
     x = importlib.import_module("parallel-critical-collapse_amd.workload").initial_x(shape)
     name = ("mixed" if precision == cc.CC_MIXED else "fp64") + ("+reuse" if reuse else "")
+    tangent = precision != cc.CC_FP64
     tuned = None
     if isinstance(tune, dict):
         ctx.set_schedule(**tune)
@@ -78,16 +102,15 @@ def time_policy(cc, dev, shape, precision, steps, warmup, reuse, tune):
         ctx.newton_step_device(d_x.ptr, d_F0.ptr, d_dx.ptr)
         log(f"[bench] {name} warmup {w + 1}/{warmup}: {time.perf_counter() - t:.3f} s")
     dev.synchronize()
-    chain_ms, launches, its, nsteps = 0.0, 0, 0, 0
+    chain = np.zeros(4)
+    base = np.zeros(4)
     t0 = time.perf_counter()
     for k in range(steps):
         ctx.newton_step_device(d_x.ptr, d_F0.ptr, d_dx.ptr)
         if not sched["use_graph"]:
-            ms, nl, it, st = ctx.chain_timing()
-            chain_ms += ms
-            launches += nl
-            its += it
-            nsteps += st
+            chain += ctx.chain_timing()
+            if tangent:
+                base += ctx.base_timing()
         log(f"[bench] {name} step {k + 1}/{steps} done")
     dev.synchronize()
     wall = time.perf_counter() - t0
@@ -96,23 +119,54 @@ def time_policy(cc, dev, shape, precision, steps, warmup, reuse, tune):
         ctx.set_schedule(use_graph=0)
         ctx.profile(True)
         ctx.newton_step_device(d_x.ptr, d_F0.ptr, d_dx.ptr)
-        ms, nl, it, st = ctx.chain_timing()
-        chain_ms, launches, its, nsteps = ms * steps, nl * steps, it * steps, st * steps
+        chain = np.array(ctx.chain_timing()) * steps
+        if tangent:
+            base = np.array(ctx.base_timing()) * steps
     F0, dx = d_F0.download(), d_dx.download()
+    J = ctx.last_jacobian()
+    refine = ctx.refinement() if precision == cc.CC_MIXED else None
     ctx.close()
     return {
         "wall_s": wall, "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
-        "chain_ms": chain_ms, "launches": launches, "its": its, "irk_steps": nsteps,
-        "kbar": its / max(nsteps, 1), "F0_rms": float(np.sqrt(np.mean(F0 ** 2))), "dx": dx,
+        "chain_ms": chain[0], "launches": int(chain[1]), "its": int(chain[2]), "irk_steps": int(chain[3]),
+        "kbar": chain[2] / max(chain[3], 1), "base": base, "F0": F0, "dx": dx, "J": J, "refine": refine,
         "schedule": sched, "tuning": tuned,
     }
 
 
-def cpu_baseline(shape, threads, ncols):
+_HOST = None
+
+
+def host_info():
+    """Host threads available to this process and the CPU model, read once:
+    with OMP_PROC_BIND=close the OpenMP runtime pins the calling thread to one
+    core when it starts, so later reads of the affinity mask see one CPU."""
+    global _HOST
+    if _HOST is not None:
+        return _HOST
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    _HOST = (threads, model)
+    return _HOST
+
+
+def cpu_baseline(shape, threads, ncols, reps=3):
     """The oracle (CPU restatement, OpenMP over Jacobian columns like the
-    reference's cc_openmp build) on a bounded sample of ncols columns (rounded
-    up to a multiple of the thread count), spread over the Up, psic, fc slots
-    and Delta; scaled to Newton steps/s by (n+1)/ncols."""
+    reference's cc_openmp build, OMP_PROC_BIND=close) on a bounded sample:
+    `reps` repetitions (the reference's default, main.cpp:35) over disjoint
+    sets of ncols columns each (rounded up to a multiple of the threads),
+    spread over the Up, psic, fc slots and Delta.  value = the median
+    repetition's columns/s scaled to Newton steps/s by (n+1).  Also returns
+    the oracle's columns for the parity record."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
@@ -123,19 +177,51 @@ def cpu_baseline(shape, threads, ncols):
     W = importlib.import_module("parallel-critical-collapse_amd.workload")
     x = W.initial_x(shape)
     k = -(-max(ncols, threads) // threads) * threads
-    cols = sorted(set(int(round(v)) for v in np.linspace(0, n - 1, k)) | {shape.Ntau // 2 + 2})[:k]
-    F0 = np.zeros(n)
-    t = time.perf_counter()
-    J, status, cnt, worst = O.jacobian(cfg, x, F0, cols=cols, nthreads=threads)
-    dt = time.perf_counter() - t
-    cols_per_s = len(cols) / dt
+    allc = sorted(set(int(round(v)) for v in np.linspace(0, n - 1, reps * k)) | {shape.Ntau // 2 + 2})
+    sets = [allc[r::reps][:k] for r in range(reps)]
+    F0, st = O.Oracle(cfg).shoot(x)
+    J = np.zeros((n, n))
+    rates, secs, worst_all = [], [], 0
+    for r, cols in enumerate(sets):
+        t = time.perf_counter()
+        Jr, status, cnt, worst = O.jacobian(cfg, x, F0, cols=cols, nthreads=threads)
+        dt = time.perf_counter() - t
+        J[:, cols] = Jr[:, cols]
+        rates.append(len(cols) / dt)
+        secs.append(round(dt, 2))
+        worst_all = max(worst_all, worst)
+        log(f"[bench] cpu baseline repetition {r + 1}/{reps}: {len(cols)} columns in {dt:.1f} s")
+    med = float(np.median(rates))
+    hthreads, model = host_info()
     return {
-        "value": cols_per_s / (n + 1), "unit": "Newton steps/s", "cores": threads, "kind": "port",
-        "sample": f"{len(cols)} FD Jacobian columns spread over Up, psic, fc and Delta (full shoots, "
-                  f"Ntau={shape.Ntau}, {shape.X} x-points) in {dt:.1f} s on {threads} threads; "
-                  f"extrapolated: steps/s = columns/s / (n+1), n={n}",
-        "status_ok": bool(worst == 0), "kbar": float(cnt[1] / max(cnt[0], 1)),
-    }
+        "value": med / (n + 1), "unit": "Newton steps/s", "cores": threads, "kind": "port",
+        "sample": f"{reps} repetitions of {k} FD Jacobian columns each (disjoint sets spread over Up, psic, fc "
+                  f"and Delta; full shoots at Ntau={shape.Ntau}, {shape.X} x-points) on {threads} OpenMP threads "
+                  f"(OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}); seconds {secs}; median repetition's "
+                  f"columns/s extrapolated to steps/s = columns/s / (n+1), n={n}",
+        "host_threads": hthreads, "cpu_model": model,
+        "status_ok": bool(worst_all == 0 and st == 0),
+    }, {"F0": F0, "J": J, "cols": sorted(c for s in sets for c in s)}
+
+
+def column_parity(J, Jo, cols, shape):
+    """Jacobian columns against the oracle's: Frobenius over the sample, and
+    per column relative to max(|column|, 1e-4) (the weakest psic columns
+    carry the finite differences' own noise)."""
+    err = np.array([np.linalg.norm(J[:, c] - Jo[:, c]) for c in cols])
+    nrm = np.array([np.linalg.norm(Jo[:, c]) for c in cols])
+    e = err / np.maximum(nrm, 1e-4)
+    noise = 2.0 * math.sqrt(shape.n) * DF / shape.EpsNewton
+    return {"columns": len(cols), "frobenius_rel": float(np.sqrt(np.sum(err ** 2) / np.sum(nrm ** 2))),
+            "column_rel_median": float(np.median(e)), "column_rel_max": float(e.max()),
+            "max_ratio_to_fd_bound": float(np.max(err / (1e-5 * nrm + noise)))}
+
+
+def step_parity(dx, dxo, N):
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return {"all": rel(dx, dxo), "Up": rel(dx[:N // 4], dxo[:N // 4]),
+            "psic": rel(dx[N // 4:N // 2], dxo[N // 4:N // 2]), "fc": rel(dx[N // 2:], dxo[N // 2:])}
 
 
 def rms(v):
@@ -143,12 +229,13 @@ def rms(v):
     return float(np.sqrt(np.mean(np.asarray(v) ** 2)))
 
 
-def newton_study(cc, shape, policies, tol, max_its, tune, save_failures=None):
+def newton_study(cc, shape, policies, tol, max_its, tune, save_failures=None, schedules=None):
     """Newton from the seeded input to mismatch < tol under each precision
     policy, with the reference's damping fac = min(1, SlowError/err) and its
     divergence guard (NewtonSolver.cpp:298-358): iterations, time per
     iteration, converged Delta, and Delta's relative difference between
-    policies."""
+    policies.  schedules: policy name -> schedule fields to install instead of
+    autotuning."""
     W = importlib.import_module("parallel-critical-collapse_amd.workload")
     names = {"fp64": cc.CC_FP64, "mixed": cc.CC_MIXED, "fp64_tangent": cc.CC_FP64_TANGENT}
     dslot = shape.Ntau // 2 + 2
@@ -156,7 +243,9 @@ def newton_study(cc, shape, policies, tol, max_its, tune, save_failures=None):
     for name in policies:
         ctx = cc.Context(cc.Problem.from_shape(shape, precision=names[name]))
         x = W.initial_x(shape)
-        if tune and name != "fp64":   # FP64 finite differences keep the default schedule
+        if schedules and name in schedules:
+            ctx.set_schedule(**schedules[name])
+        elif tune and name != "fp64":   # FP64 finite differences keep the default schedule
             ctx.autotune(x)
         hist, secs = [], []
         err, err_old, increase, reached, failure = 1.0, 1.0, 0, None, None
@@ -183,7 +272,9 @@ def newton_study(cc, shape, policies, tol, max_its, tune, save_failures=None):
                 break
             x = x + min(1.0, shape.SlowError / err) * dx
             secs.append(time.perf_counter() - t)
-        out[name] = {"iterations_to_tol": reached, "mismatch": hist, "final_mismatch": hist[-1] if hist else None,
+        below = [i + 1 for i, h in enumerate(hist) if h < 1e-10]
+        out[name] = {"iterations_to_tol": reached, "evaluations_to_1e-10": below[0] if below else None,
+                     "mismatch": hist, "final_mismatch": hist[-1] if hist else None,
                      "Delta": float(x[dslot]), "s_per_iteration": float(np.mean(secs)) if secs else None,
                      "s_iterations": [round(v, 4) for v in secs], "total_s": float(np.sum(secs)),
                      "failure": failure, "schedule": ctx.schedule().as_dict()}
@@ -261,7 +352,22 @@ def batch_bench(cc, steps, warmup, tune):
         "J_rel_err_max": max(e["J_rel_err"] for e in errs),
         "J_rel_err_median": float(np.median([e["J_rel_err"] for e in errs])),
         "J_rel_err_left_side_columns_max": max(e["J_rel_err_left_side_columns"] for e in errs),
+        "F0_equal_fp64_all": all(e["F0_equal_fp64"] for e in errs),
     }
+
+
+def fp64_leg(cc, dev, shape, steps, warmup, tune):
+    """The plain FP64 form at one shape: Jacobian builds (Newton steps) per
+    second, achieved FP64 TFLOP/s of the chain kernels, algorithmic HBM GB/s."""
+    r = time_policy(cc, dev, shape, cc.CC_FP64, steps, warmup, False, tune)
+    fl = flops_chain(shape.Ntau, r["its"], r["irk_steps"])
+    ach = fl / (r["chain_ms"] / 1e3) / 1e12
+    return {"workload": f"newton_step Ntau={shape.Ntau} x_points={shape.X} n={shape.n} FP64 finite differences",
+            "jacobian_builds_per_s": round(r["steps_per_s"], 4), "ms_per_step": round(r["ms_per_step"], 2),
+            "kernel": f"k_chain<{shape.Ntau},{r['schedule']['threads_bulk']},double,float,0,2>",
+            "achieved_fp64_tflops": round(ach, 3), "peak": FP64_PEAK, "frac": round(ach / FP64_PEAK, 4),
+            "hbm_gbps_algorithmic": round(hbm_bytes_step(shape) / (r["ms_per_step"] / 1e3) / 1e9, 3),
+            "kbar": round(r["kbar"], 3), "schedule": r["schedule"]}
 
 
 def copy_calibration(nbytes):
@@ -277,6 +383,18 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": n * 4, "read": n * 4, "written": n * 4}), flush=True)
 
 
+def guarded(name, fn):
+    """An extra leg of the record: its result, or the error that stopped it."""
+    try:
+        t = time.perf_counter()
+        r = fn()
+        log(f"[bench] {name}: {time.perf_counter() - t:.1f} s")
+        return r
+    except Exception as e:      # the headline line is printed regardless
+        log(f"[bench] {name} failed: {e!r}")
+        return {"error": repr(e)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,12 +404,14 @@ def main():
                     help="workload shape 1, 2, 3 or 5; 4 = the batch of uneven problems")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 policy")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the Newton-to-convergence, Ntau=512 and batch legs")
     ap.add_argument("--no-tune", action="store_true", help="default schedule instead of the autotuner's")
     ap.add_argument("--schedule", default="", help='fixed schedule fields as JSON, e.g. \'{"threads_tangent": 128}\'; '
                     "skips the autotuner")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--cpu-columns", type=int, default=64,
-                    help="Jacobian columns of the CPU baseline sample (rounded up to a multiple of the threads)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="host threads of the CPU baseline (0: all available)")
+    ap.add_argument("--cpu-columns", type=int, default=32,
+                    help="Jacobian columns per repetition of the CPU baseline (rounded up to a multiple of the threads)")
     ap.add_argument("--newton", action="store_true",
                     help="Newton to convergence under each of --policies instead of the step benchmark")
     ap.add_argument("--policies", default="mixed,fp64_tangent,fp64")
@@ -304,12 +424,18 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
+    host_info()                 # before any OpenMP runtime starts
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the column shards of a multi-GPU run are described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    cc = importlib.import_module("parallel-critical-collapse_amd")
+    dev = importlib.import_module("parallel-critical-collapse_amd.device")
+    W = importlib.import_module("parallel-critical-collapse_amd.workload")
     if args.newton:
-        cc = importlib.import_module("parallel-critical-collapse_amd")
-        W = importlib.import_module("parallel-critical-collapse_amd.workload")
         shape = W.config(args.config)
         if args.scale is not None:
             from dataclasses import replace
@@ -320,14 +446,6 @@ def main():
                                       f"XMid={shape.XMid} scale={shape.scale}", "tol": args.tol,
                           "policies": res}), flush=True)
         return
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the column shards of a multi-GPU run are described in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
-    cc = importlib.import_module("parallel-critical-collapse_amd")
-    dev = importlib.import_module("parallel-critical-collapse_amd.device")
-    W = importlib.import_module("parallel-critical-collapse_amd.workload")
     if args.config == 4:
         print(json.dumps(batch_bench(cc, args.steps, args.warmup, not args.no_tune)), flush=True)
         return
@@ -343,25 +461,41 @@ def main():
     T = mixed["schedule"]["threads_tangent"]
     fl = flops_chain(N, mixed["its"], mixed["irk_steps"], tangent=True)
     achieved = fl / (mixed["chain_ms"] / 1e3) / 1e12
-    peak = 157.3
-    roof = {"bound": "mfma", "pipe": "fp32 valu (FFT + pointwise; no dense contraction)",
-            "kernel": f"k_tangent<{N},{T}>", "achieved": round(achieved, 3), "peak": peak,
-            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+    roof = {"bound": "valu", "pipe": "fp32 valu, non-packed (FFT passes + pointwise; no dense contraction)",
+            "kernel": f"k_tangent<{N},{T}>", "achieved": round(achieved, 3), "peak": FP32_PEAK,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK, 4),
             "traffic": MEASURED_TRAFFIC_BYTES.get((f"k_tangent<{N},{T}>", mixed["schedule"]["steps_per_launch"])),
             "flops_per_step": fl / args.steps, "kernel_s_per_step": mixed["chain_ms"] / 1e3 / args.steps,
             "avg_launch_ms": mixed["chain_ms"] / max(mixed["launches"], 1),
             "launches_per_step": mixed["launches"] / args.steps}
+    bms, bl, bits, bsteps = mixed["base"]
+    base_cus = 4                    # base and Delta column, both sides: one workgroup (one CU) each
+    roof_base = None
+    if bms > 0:
+        bfl = flops_chain(N, bits, bsteps)
+        bach = bfl / (bms / 1e3) / 1e12
+        roof_base = {"bound": "valu", "pipe": "fp64 valu; latency of dependent transform passes on 4 workgroups",
+                     "kernel": f"k_chain<{N},{mixed['schedule']['threads_base']},double,float,1,2>",
+                     "achieved": round(bach, 4), "peak": FP64_PEAK, "unit": "TFLOP/s",
+                     "frac": round(bach / FP64_PEAK, 5), "cus": base_cus,
+                     "frac_of_its_cus": round(bach / (FP64_PEAK * base_cus / CUS), 4),
+                     "kernel_ms_per_step": round(bms / args.steps, 2), "avg_launch_ms": bms / max(bl, 1),
+                     "launches_per_step": bl / args.steps, "kbar": round(bits / max(bsteps, 1), 3),
+                     "flops_per_step": bfl / args.steps}
     out = {
         "metric": "Jacobian+Newton steps/s at Ntau=1024, 8000 x-pts (mixed FP32/FP64)",
         "value": round(mixed["steps_per_s"], 5), "unit": "Newton steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(mixed["ms_per_step"], 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-        "data": "synthetic: shipped D=4 solution + seeded N(0,1)*0.1/k^2 odd/even-harmonic perturbation",
+        "data": f"synthetic: shipped D=4 solution (data/simulation_4D_{N}.json) + seeded N(0,1)*0.1/k^2 "
+                f"odd/even-harmonic perturbation scaled by {shape.scale} (the largest power of ten <= 1 at "
+                f"which the oracle's FP64 shoot survives)",
         "config": {"workload": f"newton_step Ntau={shape.Ntau} x_points={shape.X} n={shape.n} Dim={shape.Dim} "
                                f"XMid={shape.XMid} IRK2", "policy": "mixed"},
         "schedule": mixed["schedule"], "tuning": mixed["tuning"],
         "kbar_tangent": round(mixed["kbar"], 3),
-        "roofline": roof,
+        "refinement": {"residual": mixed["refine"][0], "fp64_resolve": mixed["refine"][1]},
+        "roofline": roof, "roofline_base": roof_base,
     }
     for key, res in (("fp64", fp64), ("fp64_reuse_sides", fp64r)):
         if res is None:
@@ -369,14 +503,64 @@ def main():
         fl64 = flops_chain(N, res["its"], res["irk_steps"])
         ach = fl64 / (res["chain_ms"] / 1e3) / 1e12
         out[key] = {"value": round(res["steps_per_s"], 5), "ms_per_step": round(res["ms_per_step"], 2),
-                    "kbar": round(res["kbar"], 3), "kernel": f"k_chain<{N},{res['schedule']['threads_bulk']},double>",
-                    "achieved_tflops": round(ach, 3), "peak": 78.6, "frac": round(ach / 78.6, 4),
+                    "kbar": round(res["kbar"], 3),
+                    "kernel": f"k_chain<{N},{res['schedule']['threads_bulk']},double,float,0,2>",
+                    "achieved_tflops": round(ach, 3), "peak": FP64_PEAK, "frac": round(ach / FP64_PEAK, 4),
                     "avg_launch_ms": res["chain_ms"] / max(res["launches"], 1),
                     "launches_per_step": res["launches"] / args.steps, "schedule": res["schedule"]}
         out[f"mixed_over_{key}"] = round(mixed["steps_per_s"] / res["steps_per_s"], 3)
+
+    # parity against the oracle at this shape: its step (fixture) and its columns (CPU leg)
+    par = {}
+    fxp = os.path.join(GOLDEN, f"step_c{args.config}_oracle.json")
+    if os.path.exists(fxp):
+        with open(fxp) as f:
+            fx = json.load(f)
+        if fx["shape"]["Ntau"] == N and fx["shape"]["x_points"] == shape.X and fx["shape"]["seed"] == shape.seed:
+            dxo = np.array(fx["dx"])
+            par["dx_vs_oracle_step"] = {"mixed": step_parity(mixed["dx"], dxo, N)}
+            par["F0_vs_oracle_max_abs"] = float(np.max(np.abs(mixed["F0"] - np.array(fx["F0"]))))
+            par["oracle_cond_J"] = fx["sigma_max"] / fx["sigma_min"]
+            if fp64 is not None:
+                par["dx_vs_oracle_step"]["fp64"] = step_parity(fp64["dx"], dxo, N)
+    for key, res in (("mixed", mixed), ("fp64", fp64)):
+        if res is not None:
+            par.setdefault("step_residual", {})[key] = float(np.linalg.norm(res["J"] @ res["dx"] + res["F0"]) /
+                                                             np.linalg.norm(res["F0"]))
     if not args.no_cpu:
-        log(f"[bench] cpu baseline on {args.cpu_threads} threads")
-        out["cpu_baseline"] = cpu_baseline(shape, args.cpu_threads, args.cpu_columns)
+        threads = args.cpu_threads or host_info()[0]
+        log(f"[bench] cpu baseline on {threads} threads")
+        out["cpu_baseline"], ocols = cpu_baseline(shape, threads, args.cpu_columns)
+        par["columns_vs_oracle"] = {"mixed": column_parity(mixed["J"], ocols["J"], ocols["cols"], shape)}
+        if fp64 is not None:
+            par["columns_vs_oracle"]["fp64"] = column_parity(fp64["J"], ocols["J"], ocols["cols"], shape)
+    out["oracle_parity"] = par
+    if "columns_vs_oracle" in par:
+        out["J_rel_err_vs_oracle"] = {k: v["frobenius_rel"] for k, v in par["columns_vs_oracle"].items()}
+    if "dx_vs_oracle_step" in par:
+        out["dx_rel_err_vs_oracle"] = {k: v["all"] for k, v in par["dx_vs_oracle_step"].items()}
+
+    if not args.no_extras:
+        scheds = {"mixed": mixed["schedule"]}
+        if fp64 is not None:
+            scheds["fp64"] = fp64["schedule"]
+        out["newton"] = guarded("newton to convergence", lambda: dict(
+            tol=1e-12, **{"policies": newton_study(cc, shape, ["mixed", "fp64"] if fp64 is not None else ["mixed"],
+                                                   1e-12, 12, False, schedules=scheds)}))
+        out["fp64_ntau512_x6000"] = guarded("fp64 at Ntau=512", lambda: fp64_leg(
+            cc, dev, W.config(2), max(2, args.steps // 4), 1, tune))
+        if "error" not in out["fp64_ntau512_x6000"] and not args.no_cpu:
+            sh2 = W.config(2)
+            cb2, _ = cpu_baseline(sh2, args.cpu_threads or host_info()[0], 16)
+            out["fp64_ntau512_x6000"]["cpu_baseline"] = cb2
+            out["fp64_ntau512_x6000"]["gpu_over_cpu"] = round(
+                out["fp64_ntau512_x6000"]["jacobian_builds_per_s"] / cb2["value"], 1)
+        bb = guarded("batch", lambda: batch_bench(cc, 3, 1, True))
+        if "error" not in bb:
+            bb = {k: bb[k] for k in ("value", "unit", "ms_per_step", "sequential_value", "batched_over_sequential",
+                                     "chains_per_step", "J_rel_err_max", "J_rel_err_median",
+                                     "J_rel_err_left_side_columns_max", "F0_equal_fp64_all", "config")}
+        out["batch"] = bb
     print(json.dumps(out), flush=True)
 
 

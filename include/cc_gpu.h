@@ -60,8 +60,10 @@ typedef struct cc_problem {
     int32_t precision;      /* cc_precision */
     int32_t reuse_sides;    /* 1: a column that leaves one side's boundary data unchanged
                                reuses the base shoot of that side (bit-identical result) */
-    double  PrecisionIRK32; /* IRK tolerance of FP32 shoots (0 = default 2e-6) */
-    double  EpsNewton32;    /* FD step of FP32 columns (0 = default 1e-3) */
+    double  PrecisionIRK32; /* IRK tolerance of FP32 chains in cc_integrate (0 = default 2e-6) */
+    double  refine_tol;     /* CC_MIXED: relative residual ||J dx - b|| / ||b|| the FP64-refined
+                               solve must reach; above it (or not finite) the system is solved
+                               again with the FP64 LU (0 = default 1e-10) */
     int32_t refine_iters;   /* FP64 iterative-refinement sweeps of the Newton step (0 = default 3) */
     int32_t reserved;
 } cc_problem;
@@ -94,8 +96,18 @@ cc_status cc_jacobian(cc_ctx *ctx, const double *x, const double *F0, double *J,
  * (NewtonSolver.cpp:739-757).  A (n x n row-major, 1 <= n <= 3 Ntau/4) is
  * overwritten by its LU factors, b by the solution; *info follows LAPACK
  * (0 ok, k > 0 zero pivot at k).  In CC_MIXED the factors are FP32 (returned
- * widened) and the solution is refined in FP64. */
+ * widened) and the solution is refined in FP64; when the refined residual
+ * stays above refine_tol the system is factored again in FP64 and A
+ * receives the FP64 factors (cc_refinement reports which). */
 cc_status cc_solve(cc_ctx *ctx, int32_t n, double *A, double *b, int32_t *info);
+
+/* Outcome of the last CC_MIXED solve of this context (cc_solve,
+ * cc_newton_step*, cc_newton_run): *residual = ||J dx - b|| / ||b||
+ * after the FP64 refinement of the FP32 solve, *fp64_resolve = 1 when that
+ * exceeded refine_tol (or was not finite) and the FP64 LU solved the system
+ * instead.  Both 0 for the other policies.  No counterpart in the reference
+ * (its dgesv is FP64 throughout). */
+cc_status cc_refinement(cc_ctx *ctx, double *residual, int32_t *fp64_resolve);
 
 /* One Newton step without host round trips: F0 = F(x), J, solve J dx = -F0
  * (+ FP64 refinement in CC_MIXED).  Outputs F0 and dx (n doubles each);
@@ -118,7 +130,10 @@ cc_status cc_synchronize(cc_ctx *ctx);
  * mismatch fell; outcome 0: max_iter steps without either, x is the last
  * iterate.  *iterations is the reference's `its` at termination, *mismatch the
  * final err (errOld for outcome 2), history (may be null, max_iter doubles)
- * the mismatch of each step.  On an error status x is left unchanged. */
+ * the mismatch of each step.  On an error status x is left unchanged.  The
+ * reference tests convergence and the divergence guard before it assembles J
+ * (NewtonSolver.cpp:304-331), so a Jacobian or LU failure at an iterate whose
+ * own residual is valid and ends the loop (outcome 1 or 2) is not an error. */
 cc_status cc_newton_run(cc_ctx *ctx, double *x, double precision_newton, double slow_error, int32_t max_iter,
                         int32_t *iterations, double *mismatch, double *history, int32_t *outcome);
 
@@ -151,6 +166,10 @@ cc_status cc_counters(cc_ctx *ctx, int64_t out[3]);
  * launches, fixed-point iterations and IRK steps they executed. */
 cc_status cc_profile(cc_ctx *ctx, int32_t enable);
 cc_status cc_chain_timing(cc_ctx *ctx, double *ms, int32_t *launches, int64_t *its, int64_t *steps);
+/* The same for the FP64 base chains of the tangent-linear policies (the base
+ * shoot and the Delta column, on the second stream when pipelined): event
+ * pairs around each of their launches in the last profiled step. */
+cc_status cc_base_timing(cc_ctx *ctx, double *ms, int32_t *launches, int64_t *its, int64_t *steps);
 
 /* ---- launch schedule and autotuner (no counterpart in the reference) ----- */
 

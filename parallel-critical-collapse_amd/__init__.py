@@ -41,16 +41,16 @@ class Problem(C.Structure):
         ("Dim", C.c_double), ("XLeft", C.c_double), ("XMid", C.c_double), ("XRight", C.c_double),
         ("PrecisionIRK", C.c_double), ("EpsNewton", C.c_double),
         ("precision", C.c_int32), ("reuse_sides", C.c_int32),
-        ("PrecisionIRK32", C.c_double), ("EpsNewton32", C.c_double),
+        ("PrecisionIRK32", C.c_double), ("refine_tol", C.c_double),
         ("refine_iters", C.c_int32), ("reserved", C.c_int32),
     ]
 
     @classmethod
     def make(cls, Ntau, NLeft, NRight, Dim=4.0, XLeft=1e-3, XMid=0.5, XRight=0.999, SchemeIRK=2,
              MaxIterIRK=100, PrecisionIRK=1e-15, EpsNewton=1e-8, precision=CC_FP64, reuse_sides=0,
-             PrecisionIRK32=0.0, EpsNewton32=0.0, refine_iters=0):
+             PrecisionIRK32=0.0, refine_tol=0.0, refine_iters=0):
         return cls(Ntau, SchemeIRK, MaxIterIRK, NLeft, NRight, Dim, XLeft, XMid, XRight, PrecisionIRK,
-                   EpsNewton, precision, reuse_sides, PrecisionIRK32, EpsNewton32, refine_iters, 0)
+                   EpsNewton, precision, reuse_sides, PrecisionIRK32, refine_tol, refine_iters, 0)
 
     @classmethod
     def from_shape(cls, shape, **kw):
@@ -109,6 +109,7 @@ def load():
     L.cc_residual.argtypes = [_vp, _dp, _dp]
     L.cc_jacobian.argtypes = [_vp, _dp, _dp, _dp, _ip]
     L.cc_solve.argtypes = [_vp, C.c_int32, _dp, _dp, _ip]
+    L.cc_refinement.argtypes = [_vp, _dp, _ip]
     L.cc_newton_step.argtypes = [_vp, _dp, _dp, _dp]
     L.cc_newton_step_device_async.argtypes = [_vp, _vp, _vp, _vp]
     L.cc_synchronize.argtypes = [_vp]
@@ -118,6 +119,7 @@ def load():
     L.cc_expansions.argtypes = [_vp, _dp, _dp, _dp]
     L.cc_profile.argtypes = [_vp, C.c_int32]
     L.cc_chain_timing.argtypes = [_vp, _dp, _ip, _lp, _lp]
+    L.cc_base_timing.argtypes = [_vp, _dp, _ip, _lp, _lp]
     L.cc_shoot_fields.argtypes = [_vp, _dp, C.c_int32, C.c_int32, _ip, _dp, _dp]
     L.cc_newton_run.argtypes = [_vp, _dp, C.c_double, C.c_double, C.c_int32, _ip, _dp, _dp, _ip]
     L.cc_pack.argtypes = [_vp, _dp, _dp, C.c_int32]
@@ -147,6 +149,14 @@ def _d(a):
 
 def _arr(a, dtype=np.float64):
     return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _vec(a, n, what):
+    """A contiguous float64 vector of exactly n entries (the C side reads n)."""
+    v = np.ascontiguousarray(a, dtype=np.float64)
+    if v.ndim != 1 or v.size != n:
+        raise ValueError(f"{what}: expected {n} values, got shape {v.shape}")
+    return v
 
 
 class Context:
@@ -197,7 +207,7 @@ class Context:
     def newton_run(self, x, precision_newton, slow_error, max_iter):
         """cc_newton_run: the whole Newton loop on the device.  Returns
         (x, iterations, mismatch, history, outcome)."""
-        x = _arr(x).copy()
+        x = _vec(x, self.n, "x").copy()
         its, out = C.c_int32(0), C.c_int32(0)
         err = C.c_double(0.0)
         hist = np.zeros(max_iter)
@@ -209,7 +219,7 @@ class Context:
     def shoot_fields(self, x, every):
         """Debug field snapshots of the base shoot (cc_shoot_fields): grid
         abscissae xs [k] and fields [k][4][Ntau] = A, U, V, F."""
-        x = _arr(x)
+        x = _vec(x, self.n, "x")
         cnt = C.c_int32(0)
         st = load().cc_shoot_fields(self.h, _d(x), int(every), 0, C.byref(cnt), None, None)
         if st != CC_BADARG or cnt.value < 1:
@@ -221,13 +231,13 @@ class Context:
         return xs, fields
 
     def residual(self, x):
-        x = _arr(x)
+        x = _vec(x, self.n, "x")
         F = np.zeros(self.n)
         self._check(load().cc_residual(self.h, _d(x), _d(F)), "cc_residual")
         return F
 
     def jacobian(self, x, F0, check=True):
-        x, F0 = _arr(x), _arr(F0)
+        x, F0 = _vec(x, self.n, "x"), _vec(F0, self.n, "F0")
         J = np.zeros((self.n, self.n))
         cs = np.zeros(self.n, np.int32)
         st = load().cc_jacobian(self.h, _d(x), _d(F0), _d(J), cs.ctypes.data_as(_ip))
@@ -238,12 +248,21 @@ class Context:
     def solve(self, A, b):
         A = np.array(A, np.float64, order="C")
         b = np.array(b, np.float64)
+        if A.ndim != 2 or A.shape[0] != A.shape[1] or b.shape != (A.shape[0],):
+            raise ValueError(f"solve: A must be square and b match it, got {A.shape} and {b.shape}")
         info = C.c_int32(0)
         st = load().cc_solve(self.h, A.shape[0], _d(A), _d(b), C.byref(info))
         return b, info.value, A, st
 
+    def refinement(self):
+        """(relative residual of the last refined mixed solve, whether it was
+        solved again with the FP64 LU)."""
+        r, f = C.c_double(0.0), C.c_int32(0)
+        self._check(load().cc_refinement(self.h, C.byref(r), C.byref(f)), "cc_refinement")
+        return r.value, bool(f.value)
+
     def newton_step(self, x):
-        x = _arr(x)
+        x = _vec(x, self.n, "x")
         F0, dx = np.zeros(self.n), np.zeros(self.n)
         self._check(load().cc_newton_step(self.h, _d(x), _d(F0), _d(dx)), "cc_newton_step")
         return F0, dx
@@ -264,6 +283,15 @@ class Context:
                     "cc_chain_timing")
         return ms.value, nl.value, its.value, steps.value
 
+    def base_timing(self):
+        """(ms, launches, fixed-point iterations, IRK steps) of the FP64 base
+        chains in the last profiled step of a tangent-linear policy."""
+        ms, nl = C.c_double(0), C.c_int32(0)
+        its, steps = C.c_int64(0), C.c_int64(0)
+        self._check(load().cc_base_timing(self.h, C.byref(ms), C.byref(nl), C.byref(its), C.byref(steps)),
+                    "cc_base_timing")
+        return ms.value, nl.value, its.value, steps.value
+
     def schedule(self) -> Schedule:
         sc = Schedule()
         self._check(load().cc_get_schedule(self.h, C.byref(sc)), "cc_get_schedule")
@@ -279,7 +307,7 @@ class Context:
 
     def autotune(self, x, flags=CC_TUNE_THREADS | CC_TUNE_STAGES):
         """Run the autotuner at x; returns (installed schedule, candidate timings in ms)."""
-        x = _arr(x)
+        x = _vec(x, self.n, "x")
         sc = Schedule()
         times = np.zeros(64)
         nt = C.c_int32(0)
@@ -292,6 +320,8 @@ class Context:
 
     def fft(self, X, sign, fp32=False):
         X = _arr(np.atleast_2d(X), np.complex128)
+        if X.ndim != 2 or X.shape[1] != self.N:
+            raise ValueError(f"fft: expected signals of {self.N} values, got shape {X.shape}")
         out = np.zeros_like(X)
         self._check(load().cc_fft_batched(self.h, X.ctypes.data_as(_dp), out.ctypes.data_as(_dp),
                                           X.shape[0], sign, int(fp32)), "cc_fft_batched")
@@ -300,6 +330,8 @@ class Context:
     def rhs(self, Y, xs, fp32=False):
         Y = _arr(np.atleast_2d(Y), np.complex128)
         xs = _arr(np.atleast_1d(xs))
+        if Y.ndim != 2 or Y.shape[1] != self.M or xs.shape != (Y.shape[0],):
+            raise ValueError(f"rhs: expected states of {self.M} modes and one abscissa each, got {Y.shape}, {xs.shape}")
         dY = np.zeros_like(Y)
         self._check(load().cc_rhs_eval(self.h, Y.ctypes.data_as(_dp), _d(xs), dY.ctypes.data_as(_dp),
                                        Y.shape[0], int(fp32)), "cc_rhs_eval")
@@ -310,6 +342,9 @@ class Context:
         grid = _arr(grid)
         start = _arr(np.atleast_1d(start), np.int32)
         end = _arr(np.atleast_1d(end), np.int32)
+        if Y.ndim != 2 or Y.shape[1] != self.M or start.shape != (Y.shape[0],) or end.shape != (Y.shape[0],):
+            raise ValueError(f"integrate: expected states of {self.M} modes with a start and end each, "
+                             f"got {Y.shape}, {start.shape}, {end.shape}")
         its = np.zeros(Y.shape[0], np.int64)
         status = np.zeros(Y.shape[0], np.int32)
         self._check(load().cc_integrate(self.h, Y.ctypes.data_as(_dp), _d(grid), grid.size,
@@ -321,6 +356,8 @@ class Context:
     def pack(self, Up, psic, fc):
         """Three real tau-series (N each, or [batch, N]) -> Newton vector(s)."""
         f = _arr(np.stack([np.atleast_2d(Up), np.atleast_2d(psic), np.atleast_2d(fc)], axis=1))
+        if f.shape[2] != self.N:
+            raise ValueError(f"pack: expected series of {self.N} values, got shape {f.shape}")
         x = np.zeros((f.shape[0], self.n))
         self._check(load().cc_pack(self.h, _d(f), _d(x), f.shape[0]), "cc_pack")
         return x[0] if np.ndim(Up) == 1 else x
@@ -328,6 +365,8 @@ class Context:
     def unpack(self, x):
         """Newton vector(s) -> (Up, psic, fc), the Delta slot read as zero."""
         xb = _arr(np.atleast_2d(x))
+        if xb.ndim != 2 or xb.shape[1] != self.n:
+            raise ValueError(f"unpack: expected Newton vectors of {self.n} values, got shape {xb.shape}")
         f = np.zeros((xb.shape[0], 3, self.N))
         self._check(load().cc_unpack(self.h, _d(xb), _d(f), xb.shape[0]), "cc_unpack")
         if np.ndim(x) == 1:
@@ -335,7 +374,7 @@ class Context:
         return f[:, 0], f[:, 1], f[:, 2]
 
     def expansions(self, x):
-        x = _arr(x)
+        x = _vec(x, self.n, "x")
         YL = np.zeros(self.M, np.complex128)
         YR = np.zeros(self.M, np.complex128)
         self._check(load().cc_expansions(self.h, _d(x), YL.ctypes.data_as(_dp), YR.ctypes.data_as(_dp)),
@@ -382,7 +421,7 @@ class Batch:
 
     def newton_step(self, xs, check=True):
         """xs: list of Newton vectors -> (F0 list, dx list, per-problem status, worst status)."""
-        xs = [_arr(x) for x in xs]
+        xs = self._inputs(xs)
         F0 = [np.zeros(n) for n in self.n]
         dx = [np.zeros(n) for n in self.n]
         status = np.zeros(len(xs), np.int32)
@@ -403,8 +442,13 @@ class Batch:
         self._check(load().cc_batch_set_schedule(self.h, cls, C.byref(sc)), "cc_batch_set_schedule")
         return sc
 
+    def _inputs(self, xs):
+        if len(xs) != len(self.n):
+            raise ValueError(f"expected {len(self.n)} Newton vectors, got {len(xs)}")
+        return [_vec(x, n, f"x[{i}]") for i, (x, n) in enumerate(zip(xs, self.n))]
+
     def autotune(self, xs):
-        xs = [_arr(x) for x in xs]
+        xs = self._inputs(xs)
         times = np.zeros(128)
         nt = C.c_int32(0)
         self._check(load().cc_batch_autotune(self.h, _ptrs(xs), _d(times), times.size, C.byref(nt)),

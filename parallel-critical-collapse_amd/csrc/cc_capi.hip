@@ -76,7 +76,8 @@ struct StatusBlock {
     int32_t *tstatus;            // tangent chains
     int64_t *tits;
     int32_t *info;               // LU
-    int32_t *newton;             // device Newton loop: [0] outcome of the last update
+    int32_t *newton;             // device Newton loop: [0] outcome, [1] keep, [2] applied (last update)
+    double *refine;              // relative residual of the refined mixed solve
 };
 
 struct cc_ctx {
@@ -99,8 +100,12 @@ struct cc_ctx {
     double *A = nullptr;                 // n x n LU work (FP64)
     float *A32 = nullptr;                // n x n LU work (FP32)
     int32_t *perm = nullptr, *lu_swap = nullptr, *info = nullptr;   // LU row permutation, panel swap list
-    double *xold = nullptr, *nst = nullptr;  // device Newton loop: last improving x; err, errOld
-    int32_t *nist = nullptr;                 // device Newton loop: rises, outcome
+    double *xold = nullptr, *nst = nullptr;  // device Newton loop: last improving x; err, errOld, damping
+    int32_t *nist = nullptr;                 // device Newton loop: rises, outcome, keep, applied
+    double *refres = nullptr;                // relative residual of the last refined mixed solve
+    int32_t *rhold = nullptr;                // 1: that solve is to be redone with the FP64 LU
+    double last_refine = 0.0;                // host copies of the last mixed solve's outcome
+    int32_t last_resolve = 0;
     ColumnBuffers main;                  // base + every column (FP64 Jacobian)
     ColumnBuffers base;                  // base + Delta column (residual, mixed policy)
     Plan main_plan, base_plan;
@@ -130,6 +135,11 @@ struct cc_ctx {
     int32_t prof_launches = 0;
     int64_t prof_its = 0, prof_steps = 0;
     bool prof_pending = false;
+    // the same for the FP64 base chains of the tangent policies (their own stream when pipelined)
+    std::vector<hipEvent_t> bev;
+    int bprof_pairs = 0;
+    int32_t bprof_launches = 0;
+    int64_t bprof_its = 0, bprof_steps = 0;
     // captured Newton step
     hipGraphExec_t gexec = nullptr;
     const void *gkey[3] = {nullptr, nullptr, nullptr};
@@ -205,6 +215,33 @@ hipError_t dalloc(cc_ctx *c, Tp **p, size_t count)
     }
     return e;
 }
+
+// Device buffers of one call: release() (or the destructor, on an early
+// return) waits for the call's stream and frees every buffer.
+struct CallBuffers {
+    hipStream_t s;
+    std::vector<void *> p;
+    template <typename Tp>
+    hipError_t get(Tp **out, size_t count)
+    {
+        void *q = nullptr;
+        const hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(Tp));
+        if (e == hipSuccess) p.push_back(q);
+        *out = static_cast<Tp *>(q);
+        return e;
+    }
+    hipError_t release()
+    {
+        hipError_t e = p.empty() ? hipSuccess : hipStreamSynchronize(s);
+        for (void *q : p) {
+            const hipError_t ef = hipFree(q);
+            if (e == hipSuccess) e = ef;
+        }
+        p.clear();
+        return e;
+    }
+    ~CallBuffers() { (void)release(); }
+};
 
 template <typename Tp>
 cc_status fill(cc_ctx *c, Tp *p, Tp v, size_t count, hipStream_t s)
@@ -335,8 +372,25 @@ cc_status time_mark(cc_ctx *c, hipStream_t s, bool begin)
     return CC_OK;
 }
 
+// Event pair around one base-chain launch of the tangent policies, on its stream s.
+cc_status time_mark_base(cc_ctx *c, hipStream_t s, bool begin)
+{
+    if (!c->profile || c->capturing) return CC_OK;
+    const size_t idx = 2 * (size_t)c->bprof_pairs + (begin ? 0 : 1);
+    while (c->bev.size() <= idx) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreate(&e));
+        c->bev.push_back(e);
+    }
+    HIPCHK(c, hipEventRecord(c->bev[idx], s));
+    if (!begin) ++c->bprof_pairs;
+    return CC_OK;
+}
+
 void time_reset(cc_ctx *c)
 {
+    c->bprof_pairs = 0;
+    c->bprof_launches = 0;
     c->prof_pairs = 0;
     c->prof_launches = 0;
 }
@@ -470,31 +524,39 @@ cc_status eval_columns(cc_ctx *c, const Plan &P, Counts k, bool is_base, int32_t
 // ---------------------------------------------------------------------------
 // Dense solve: LU with partial pivoting (FP64, or FP32 + FP64 refinement)
 // ---------------------------------------------------------------------------
+// The factorization's launches on s: perm (n ints), swap (1 + 2 LU_SWAP_MAX
+// ints) and info (1 int, cleared by the caller) are its scratch; n <= LU_MAXN.
 template <typename R>
-cc_status lu_factor(cc_ctx *c, R *dA, int n, hipStream_t s)
+hipError_t lu_factor_on(R *dA, int n, int32_t *perm, int32_t *swap, int32_t *info, hipStream_t s)
 {
     constexpr int NB = ccd::lu_nb<R>();
-    if (n > ccd::LU_MAXN) {
-        c->err = "LU order above " + std::to_string(ccd::LU_MAXN);
-        return CC_BADARG;
-    }
     const unsigned colblocks = (unsigned)((n + 63) / 64);
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int m = n - k0, nb = std::min(NB, m), rest = m - nb;
         if (m <= ccd::LU_PANEL_ROWS)
-            hipLaunchKernelGGL((ccd::k_lu_panel<R, true>), dim3(1), dim3(ccd::LU_PANEL_T), 0, s, dA, n, k0, c->perm,
-                               c->lu_swap, c->info);
+            hipLaunchKernelGGL((ccd::k_lu_panel<R, true>), dim3(1), dim3(ccd::LU_PANEL_T), 0, s, dA, n, k0, perm,
+                               swap, info);
         else
-            hipLaunchKernelGGL((ccd::k_lu_panel<R, false>), dim3(1), dim3(ccd::LU_PANEL_T), 0, s, dA, n, k0, c->perm,
-                               c->lu_swap, c->info);
-        hipLaunchKernelGGL((ccd::k_lu_laswp<R>), dim3(colblocks), dim3(256), 0, s, dA, n, k0, nb, c->lu_swap);
+            hipLaunchKernelGGL((ccd::k_lu_panel<R, false>), dim3(1), dim3(ccd::LU_PANEL_T), 0, s, dA, n, k0, perm,
+                               swap, info);
+        hipLaunchKernelGGL((ccd::k_lu_laswp<R>), dim3(colblocks), dim3(256), 0, s, dA, n, k0, nb, swap);
         if (rest > 0) {
             hipLaunchKernelGGL((ccd::k_lu_trsm<R>), dim3((rest + 255) / 256), dim3(256), 0, s, dA, n, k0);
             const int tiles = (rest + 63) / 64;
             hipLaunchKernelGGL((ccd::k_lu_gemm<R>), dim3(tiles, tiles), dim3(256), 0, s, dA, n, k0, nb);
         }
     }
-    HIPCHK(c, hipGetLastError());
+    return hipGetLastError();
+}
+
+template <typename R>
+cc_status lu_factor(cc_ctx *c, R *dA, int n, hipStream_t s)
+{
+    if (n > ccd::LU_MAXN) {
+        c->err = "LU order above " + std::to_string(ccd::LU_MAXN);
+        return CC_BADARG;
+    }
+    HIPCHK(c, lu_factor_on<R>(dA, n, c->perm, c->lu_swap, c->info, s));
     return CC_OK;
 }
 
@@ -521,9 +583,42 @@ cc_status enqueue_solve(cc_ctx *c, int n, const double *d_J, const double *d_rhs
                                c->r);
             hipLaunchKernelGGL(ccd::k_axpy, dim3((n + 255) / 256), dim3(256), 0, s, d_sol, c->r, n);
         }
+        // how far the refinement got: ||rhs - J sol|| / ||rhs||, read by the host
+        // after the step and by a device Newton update that must not use a bad sol
+        hipLaunchKernelGGL(ccd::k_residual, dim3(n), dim3(256), 0, s, d_J, d_sol, d_rhs, c->r, n);
+        hipLaunchKernelGGL(ccd::k_refine_check, dim3(1), dim3(256), 0, s, c->r, d_rhs, c->info, n, c->p.refine_tol,
+                           c->refres, c->rhold);
+        HIPCHK(c, hipMemcpyAsync(c->hs.refine, c->refres, sizeof(double), hipMemcpyDeviceToHost, s));
     }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->hs.info, c->info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    return CC_OK;
+}
+
+// After the stream of a CC_MIXED solve synchronised: when the FP64-refined
+// FP32 solve of J sol = rhs left a relative residual above refine_tol (or a
+// non-finite one, or the FP32 factorization met a zero pivot), the system is
+// solved again with the FP64 LU, synchronously on s, and *resolved is set.
+// The LU info of the solve that stands is in hs.info afterwards.
+cc_status refine_or_resolve(cc_ctx *c, int n, const double *d_J, const double *d_rhs, double *d_sol, hipStream_t s,
+                            bool *resolved)
+{
+    *resolved = false;
+    c->last_refine = 0.0;
+    c->last_resolve = 0;
+    if (c->p.precision != CC_MIXED) return CC_OK;
+    c->last_refine = c->hs.refine[0];
+    if (c->hs.refine[0] <= c->p.refine_tol && c->hs.info[0] == 0) return CC_OK;
+    CCCHK(fill<int32_t>(c, c->info, 0, 1, s));
+    HIPCHK(c, hipMemcpyAsync(c->A, d_J, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, s));
+    CCCHK(lu_factor<double>(c, c->A, n, s));
+    HIPCHK(c, hipMemcpyAsync(d_sol, d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL((ccd::k_lu_solve<double, double>), dim3(1), dim3(1024), 0, s, c->A, n, c->perm, d_sol);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->hs.info, c->info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->last_resolve = 1;
+    *resolved = true;
     return CC_OK;
 }
 
@@ -630,7 +725,10 @@ cc_status enqueue_tangent_n(cc_ctx *c, const double *d_x)
     for (int k = 0; k < nchunks; ++k) {
         ab.step_begin = at.step_begin = k * S;
         ab.step_end = at.step_end = std::min(maxsteps, (k + 1) * S);
+        CCCHK(time_mark_base(c, sb, true));
         HIPCHK(c, Ops<N>::chain(sb, c->sched.threads_base, kb.nch, ab, t64));
+        CCCHK(time_mark_base(c, sb, false));
+        ++c->bprof_launches;
         if (pipe) {
             HIPCHK(c, hipEventRecord(c->chunk_ev[k], sb));
             HIPCHK(c, hipStreamWaitEvent(s0, c->chunk_ev[k], 0));
@@ -686,6 +784,12 @@ cc_status eval_tangent(cc_ctx *c, int32_t *col_status)
     c->counters[2] += nt;
     c->prof_its = itsum;
     c->prof_steps = steps;
+    c->bprof_its = c->bprof_steps = 0;
+    const Counts kb = counts_of(c->base_plan);
+    for (int i = 0; i < kb.nch; ++i) {
+        c->bprof_its += c->hs.bits[i];
+        c->bprof_steps += c->base_plan.nsteps[i];
+    }
     if (col_status) {
         for (int i = 0; i < c->n; ++i) col_status[i] = sb;
         for (int b = 0; b < nt; ++b)
@@ -757,9 +861,12 @@ void drop_graph(cc_ctx *c)
 
 // post (may be null): more work enqueued on s0 behind the step, before the
 // step's one host synchronisation.
+// resolved (may be null): set when the mixed solve was redone in FP64 after
+// the step's own work (and after post), so that d_dx changed since.
 cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double *d_dx,
-                             const std::function<cc_status()> *post = nullptr)
+                             const std::function<cc_status()> *post = nullptr, bool *resolved = nullptr)
 {
+    if (resolved) *resolved = false;
     c->counters[0] = c->counters[1] = c->counters[2] = 0;
     time_reset(c);
     if (c->sched.use_graph) {
@@ -796,7 +903,22 @@ cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double 
     HIPCHK(c, hipStreamSynchronize(c->s0));
     const cc_status st = tangent_policy(c) ? eval_tangent(c, nullptr) : eval_fp64(c, nullptr);
     CCCHK(st);
+    bool res = false;
+    CCCHK(refine_or_resolve(c, c->n, c->J, c->d, d_dx, c->s0, &res));
+    if (resolved) *resolved = res;
     return eval_info(c, nullptr);
+}
+
+// Whether the base column (index 0 of both plans) of the last step ran
+// cleanly, so that F0 is a valid residual whatever the other columns did.
+bool base_residual_ok(const cc_ctx *c)
+{
+    if (tangent_policy(c)) {
+        const Plan &P = c->base_plan;
+        return c->hs.bstatus[P.lidx[0]] == 0 && c->hs.bstatus[P.ridx[0]] == 0 && c->hs.bmstatus[0] == 0;
+    }
+    const Plan &P = c->main_plan;
+    return c->hs.status[P.lidx[0]] == 0 && c->hs.status[P.ridx[0]] == 0 && c->hs.mstatus[0] == 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1150,6 +1272,8 @@ cc_status batch_run(cc_batch *b, int32_t *status)
         cc_ctx *c = b->ctx[i];
         c->counters[0] = c->counters[1] = c->counters[2] = 0;
         cc_status st = eval_tangent(c, nullptr);
+        bool res = false;
+        if (st == CC_OK) st = refine_or_resolve(c, c->n, c->J, c->d, c->dx, c->s0, &res);
         if (st == CC_OK) st = eval_info(c, nullptr);
         for (int k = 0; k < 3; ++k) b->counters[k] += c->counters[k];
         if (status) status[i] = st;
@@ -1195,7 +1319,7 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
     cc_ctx *c = new cc_ctx();
     c->p = *p;
     if (c->p.PrecisionIRK32 <= 0.0) c->p.PrecisionIRK32 = 2e-6;
-    if (c->p.EpsNewton32 <= 0.0) c->p.EpsNewton32 = 1e-3;
+    if (c->p.refine_tol <= 0.0) c->p.refine_tol = 1e-10;
     if (c->p.refine_iters <= 0) c->p.refine_iters = 3;
     if (p->precision == CC_FP64_TANGENT) c->tangent_rtol = N >= 2048 ? 1e-10 : 1e-13;
     c->N = N;
@@ -1304,8 +1428,10 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
     ALLOC(c->A32, (size_t)n * n);
     ALLOC(c->perm, n);
     ALLOC(c->xold, n);
-    ALLOC(c->nst, 2);
+    ALLOC(c->nst, 3);
     ALLOC(c->nist, 4);
+    ALLOC(c->refres, 1);
+    ALLOC(c->rhold, 1);
     ALLOC(c->lu_swap, 1 + 2 * ccd::LU_SWAP_MAX);
     ALLOC(c->info, 1);
     for (ColumnBuffers *B : {&c->main, &c->base}) {
@@ -1343,13 +1469,15 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
     // pinned status block
     {
         const size_t mch = c->maxchains, mc = c->maxcols;
-        const size_t bytes = sizeof(int64_t) * (mch + 4 + (size_t)n) + sizeof(int32_t) * (mch + mc + 4 + 2 + n + 2 + 4);
+        const size_t bytes = sizeof(double) * 2 + sizeof(int64_t) * (mch + 4 + (size_t)n) +
+                             sizeof(int32_t) * (mch + mc + 4 + 2 + n + 2 + 4);
         if (bad(hipHostMalloc(&c->hblock, bytes, hipHostMallocDefault), "hipHostMalloc")) return CC_DEVICE;
         char *q = static_cast<char *>(c->hblock);
         auto take = [&](auto **ptr, size_t cnt) {
             *ptr = reinterpret_cast<std::remove_pointer_t<decltype(ptr)>>(q);
             q += cnt * sizeof(**ptr);
         };
+        take(&c->hs.refine, 2);
         take(&c->hs.its, mch);
         take(&c->hs.bits, 4);
         take(&c->hs.tits, n);
@@ -1435,6 +1563,7 @@ cc_status cc_ctx_destroy(cc_ctx *c)
     if (c->hblock) (void)hipHostFree(c->hblock);
     for (hipEvent_t e : c->chunk_ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->bev) (void)hipEventDestroy(e);
     for (hipEvent_t e : {c->ev_fork, c->ev_join, c->tk0, c->tk1}) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->s0);
     (void)hipStreamDestroy(c->s1);
@@ -1466,6 +1595,14 @@ cc_status cc_counters(cc_ctx *c, int64_t out[3])
     return CC_OK;
 }
 
+cc_status cc_refinement(cc_ctx *c, double *residual, int32_t *fp64_resolve)
+{
+    if (!c || !residual || !fp64_resolve) return CC_BADARG;
+    *residual = c->last_refine;
+    *fp64_resolve = c->last_resolve;
+    return CC_OK;
+}
+
 cc_status cc_profile(cc_ctx *c, int32_t enable)
 {
     if (!c) return CC_BADARG;
@@ -1492,6 +1629,27 @@ cc_status cc_chain_timing(cc_ctx *c, double *ms, int32_t *launches, int64_t *its
     *launches = c->prof_launches;
     *its = c->prof_its;
     *steps = c->prof_steps;
+    return CC_OK;
+}
+
+cc_status cc_base_timing(cc_ctx *c, double *ms, int32_t *launches, int64_t *its, int64_t *steps)
+{
+    if (!c || !ms || !launches || !its || !steps) return CC_BADARG;
+    if (!c->prof_pending || !tangent_policy(c)) {
+        c->err = "no profiled eager tangent-policy step since cc_profile(ctx, 1)";
+        return CC_BADARG;
+    }
+    double tot = 0.0;
+    for (int i = 0; i < c->bprof_pairs; ++i) {
+        HIPCHK(c, hipEventSynchronize(c->bev[2 * i + 1]));
+        float f = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&f, c->bev[2 * i], c->bev[2 * i + 1]));
+        tot += f;
+    }
+    *ms = tot;
+    *launches = c->bprof_launches;
+    *its = c->bprof_its;
+    *steps = c->bprof_steps;
     return CC_OK;
 }
 
@@ -1609,7 +1767,10 @@ cc_status cc_solve(cc_ctx *c, int32_t n, double *A, double *b, int32_t *info)
     HIPCHK(c, hipMemcpyAsync(c->J, A, sizeof(double) * n * n, hipMemcpyHostToDevice, c->s0));
     HIPCHK(c, hipMemcpyAsync(c->d, b, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
     CCCHK(enqueue_solve(c, n, c->J, c->d, c->dx, c->s0));
-    if (c->p.precision == CC_MIXED) {
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    bool resolved = false;
+    CCCHK(refine_or_resolve(c, n, c->J, c->d, c->dx, c->s0, &resolved));
+    if (c->p.precision == CC_MIXED && !resolved) {
         // FP32 factors, returned widened to double
         const unsigned g = (unsigned)(((size_t)n * n + 255) / 256);
         hipLaunchKernelGGL((ccd::k_cast_from<float>), dim3(g), dim3(256), 0, c->s0, c->A32, c->A, (size_t)n * n);
@@ -1660,13 +1821,36 @@ cc_status cc_newton_run(cc_ctx *c, double *x, double precision_newton, double sl
     for (its = 0; its < max_iter; ++its) {
         const std::function<cc_status()> post = [&]() -> cc_status {
             hipLaunchKernelGGL(ccd::k_newton_update, dim3(1), dim3(256), 0, c->s0, c->x, c->xold, c->F0, c->dx, n,
-                               precision_newton, slow_error, c->nst, c->nist, hist, its);
+                               precision_newton, slow_error, c->nst, c->nist, hist, its,
+                               c->p.precision == CC_MIXED ? (const int32_t *)c->rhold : nullptr);
             HIPCHK(c, hipGetLastError());
-            HIPCHK(c, hipMemcpyAsync(c->hs.newton, c->nist + 1, sizeof(int32_t), hipMemcpyDeviceToHost, c->s0));
+            HIPCHK(c, hipMemcpyAsync(c->hs.newton, c->nist + 1, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, c->s0));
             return CC_OK;
         };
-        const cc_status st = newton_step_device(c, c->x, c->F0, c->dx, &post);
-        if (st != CC_OK) return done(st);
+        bool resolved = false;
+        const cc_status st = newton_step_device(c, c->x, c->F0, c->dx, &post, &resolved);
+        if (st != CC_OK) {
+            // a failure after the residual (Jacobian columns, LU): the reference
+            // has already stopped there when the iterate converged or the
+            // mismatch rose a fifth time (NewtonSolver.cpp:304-331)
+            const bool late = st == CC_IRK_NOCONV || st == CC_DELTA_MISMATCH || st == CC_SINGULAR;
+            if (late && base_residual_ok(c) && (c->hs.newton[0] == 1 || c->hs.newton[0] == 2)) {
+                mode = c->hs.newton[0];
+                break;
+            }
+            return done(st);
+        }
+        if (resolved && c->hs.newton[0] == 0 && c->hs.newton[2] == 0) {
+            // the update was held back for the FP64 re-solve of the step
+            hipLaunchKernelGGL(ccd::k_newton_apply, dim3(1), dim3(256), 0, c->s0, c->x, c->xold, c->dx, n, c->nst,
+                               c->nist);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(c->s0);
+            if (e != hipSuccess) {
+                c->err = std::string("hip: ") + hipGetErrorString(e);
+                return done(CC_DEVICE);
+            }
+        }
         mode = c->hs.newton[0];
         if (mode != 0) break;
     }
@@ -1705,9 +1889,10 @@ cc_status cc_fft_batched(cc_ctx *c, const double *in, double *out, int32_t batch
     if (!c || !in || !out || batch < 1 || (sign != 1 && sign != -1)) return CC_BADARG;
     const int N = c->N;
     const size_t bytes = sizeof(double) * 2 * (size_t)N * batch;
+    CallBuffers tmp{c->s0, {}};
     cx<double> *din = nullptr, *dout = nullptr;
-    HIPCHK(c, hipMalloc(&din, bytes));
-    HIPCHK(c, hipMalloc(&dout, bytes));
+    HIPCHK(c, tmp.get(&din, (size_t)N * batch));
+    HIPCHK(c, tmp.get(&dout, (size_t)N * batch));
     HIPCHK(c, hipMemcpyAsync(din, in, bytes, hipMemcpyHostToDevice, c->s0));
     hipError_t e = hipSuccess;
     dispatch_n(N, [&](auto Nc) {
@@ -1716,9 +1901,7 @@ cc_status cc_fft_batched(cc_ctx *c, const double *in, double *out, int32_t batch
     });
     HIPCHK(c, e);
     HIPCHK(c, hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, c->s0));
-    HIPCHK(c, hipStreamSynchronize(c->s0));
-    HIPCHK(c, hipFree(din));
-    HIPCHK(c, hipFree(dout));
+    HIPCHK(c, tmp.release());
     return CC_OK;
 }
 
@@ -1736,13 +1919,14 @@ cc_status cc_rhs_eval(cc_ctx *c, const double *Y, const double *xs, double *dY, 
         }
     }
     const size_t nY = (size_t)2 * batch * M;
-    void *dYin = nullptr, *dYout = nullptr;
+    CallBuffers tmp{c->s0, {}};
+    cx<double> *dYin = nullptr, *dYout = nullptr;
     double *dx_ = nullptr, *dd = nullptr, *dtmp = nullptr;
-    HIPCHK(c, hipMalloc(&dYin, nY * sizeof(cx<double>)));
-    HIPCHK(c, hipMalloc(&dYout, nY * sizeof(cx<double>)));
-    HIPCHK(c, hipMalloc(&dtmp, nY * sizeof(cx<double>)));
-    HIPCHK(c, hipMalloc(&dx_, sizeof(double) * 2 * batch));
-    HIPCHK(c, hipMalloc(&dd, sizeof(double) * 2 * batch));
+    HIPCHK(c, tmp.get(&dYin, nY));
+    HIPCHK(c, tmp.get(&dYout, nY));
+    HIPCHK(c, tmp.get(&dtmp, 2 * nY));
+    HIPCHK(c, tmp.get(&dx_, 2 * (size_t)batch));
+    HIPCHK(c, tmp.get(&dd, 2 * (size_t)batch));
     HIPCHK(c, hipMemcpyAsync(dtmp, Yp.data(), nY * sizeof(cx<double>), hipMemcpyHostToDevice, c->s0));
     HIPCHK(c, hipMemcpyAsync(dx_, xp.data(), sizeof(double) * 2 * batch, hipMemcpyHostToDevice, c->s0));
     HIPCHK(c, hipMemcpyAsync(dd, dp.data(), sizeof(double) * 2 * batch, hipMemcpyHostToDevice, c->s0));
@@ -1752,11 +1936,11 @@ cc_status cc_rhs_eval(cc_ctx *c, const double *Y, const double *xs, double *dY, 
         using O = Ops<decltype(Nc)::value>;
         if (fp32) {
             hipLaunchKernelGGL((ccd::k_cast_to<float>), dim3(g), dim3(256), 0, c->s0, dtmp, (float *)dYin, 2 * nY);
-            e = O::rhs_pairs(c->s0, true, batch, dYin, dx_, dd, dYout, c->tw32, c->invm32, c->p.Dim);
+            e = O::rhs_pairs(c->s0, true, batch, (void *)dYin, dx_, dd, (void *)dYout, c->tw32, c->invm32, c->p.Dim);
             hipLaunchKernelGGL((ccd::k_cast_from<float>), dim3(g), dim3(256), 0, c->s0, (const float *)dYout, dtmp,
                                2 * nY);
         } else {
-            e = O::rhs_pairs(c->s0, false, batch, dtmp, dx_, dd, dYout, c->tw64, c->invm64, c->p.Dim);
+            e = O::rhs_pairs(c->s0, false, batch, (void *)dtmp, dx_, dd, (void *)dYout, c->tw64, c->invm64, c->p.Dim);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(dtmp, dYout, nY * sizeof(cx<double>), hipMemcpyDeviceToDevice, c->s0);
         }
@@ -1768,7 +1952,7 @@ cc_status cc_rhs_eval(cc_ctx *c, const double *Y, const double *xs, double *dY, 
     HIPCHK(c, hipStreamSynchronize(c->s0));
     for (int b = 0; b < batch; ++b)
         std::memcpy(dY + (size_t)b * 2 * M, &outp[(size_t)(2 * b) * 2 * M], sizeof(double) * 2 * M);
-    for (void *q : {dYin, dYout, (void *)dx_, (void *)dd, (void *)dtmp}) HIPCHK(c, hipFree(q));
+    HIPCHK(c, tmp.release());
     return CC_OK;
 }
 
@@ -1910,20 +2094,21 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
         maxsteps = std::max(maxsteps, ns[b]);
     }
     const size_t nY = (size_t)batch * M;
+    CallBuffers tmp{c->s0, {}};         // freed on every return below
     double *dg = nullptr, *dd = nullptr, *dtmp = nullptr;
-    void *dYc = nullptr;
+    float *dYc = nullptr;
     int32_t *dst = nullptr, *dns = nullptr, *ddr = nullptr, *dstat = nullptr, *dfail = nullptr;
     int64_t *dits = nullptr;
-    HIPCHK(c, hipMalloc(&dg, sizeof(double) * ngrid));
-    HIPCHK(c, hipMalloc(&dd, sizeof(double) * batch));
-    HIPCHK(c, hipMalloc(&dtmp, sizeof(cx<double>) * nY));
-    HIPCHK(c, hipMalloc(&dYc, sizeof(cx<double>) * nY));
-    HIPCHK(c, hipMalloc(&dst, sizeof(int32_t) * batch));
-    HIPCHK(c, hipMalloc(&dns, sizeof(int32_t) * batch));
-    HIPCHK(c, hipMalloc(&ddr, sizeof(int32_t) * batch));
-    HIPCHK(c, hipMalloc(&dstat, sizeof(int32_t) * batch));
-    HIPCHK(c, hipMalloc(&dfail, sizeof(int32_t) * batch));
-    HIPCHK(c, hipMalloc(&dits, sizeof(int64_t) * batch));
+    HIPCHK(c, tmp.get(&dg, ngrid));
+    HIPCHK(c, tmp.get(&dd, batch));
+    HIPCHK(c, tmp.get(&dtmp, 2 * nY));
+    HIPCHK(c, tmp.get(&dYc, 2 * nY));
+    HIPCHK(c, tmp.get(&dst, batch));
+    HIPCHK(c, tmp.get(&dns, batch));
+    HIPCHK(c, tmp.get(&ddr, batch));
+    HIPCHK(c, tmp.get(&dstat, batch));
+    HIPCHK(c, tmp.get(&dfail, batch));
+    HIPCHK(c, tmp.get(&dits, batch));
     HIPCHK(c, hipMemcpyAsync(dg, grid, sizeof(double) * ngrid, hipMemcpyHostToDevice, c->s0));
     HIPCHK(c, hipMemcpyAsync(dd, dl.data(), sizeof(double) * batch, hipMemcpyHostToDevice, c->s0));
     HIPCHK(c, hipMemcpyAsync(dtmp, Y, sizeof(cx<double>) * nY, hipMemcpyHostToDevice, c->s0));
@@ -1939,7 +2124,7 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
         auto run = [&](auto tag) {
             using R = decltype(tag);
             ccd::ChainParams<R> a{};
-            a.Y = static_cast<cx<R> *>(sizeof(R) == 8 ? (void *)dtmp : dYc);
+            a.Y = static_cast<cx<R> *>(sizeof(R) == 8 ? (void *)dtmp : (void *)dYc);
             a.start = dst;
             a.nsteps = dns;
             a.dir = ddr;
@@ -1963,7 +2148,7 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
             }
         };
         if (fp32) {
-            hipLaunchKernelGGL((ccd::k_cast_to<float>), dim3(g), dim3(256), 0, c->s0, dtmp, (float *)dYc, 2 * nY);
+            hipLaunchKernelGGL((ccd::k_cast_to<float>), dim3(g), dim3(256), 0, c->s0, dtmp, dYc, 2 * nY);
             run(float{});
             hipLaunchKernelGGL((ccd::k_cast_from<float>), dim3(g), dim3(256), 0, c->s0, (const float *)dYc, dtmp,
                                2 * nY);
@@ -1983,9 +2168,7 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
         for (int b = 0; b < batch; ++b) status[b] = stv[b];
     if (its_total)
         for (int b = 0; b < batch; ++b) its_total[b] = itv[b];
-    for (void *q : {(void *)dg, (void *)dd, (void *)dtmp, dYc, (void *)dst, (void *)dns, (void *)ddr, (void *)dstat,
-                    (void *)dfail, (void *)dits})
-        HIPCHK(c, hipFree(q));
+    HIPCHK(c, tmp.release());
     return CC_OK;
 }
 
@@ -1993,9 +2176,10 @@ static cc_status pack_unpack(cc_ctx *c, const double *in, double *out, int32_t b
 {
     if (!c || !in || !out || batch < 1) return CC_BADARG;
     const size_t nin = (size_t)batch * (pack ? 3 * c->N : c->n), nout = (size_t)batch * (pack ? c->n : 3 * c->N);
+    CallBuffers tmp{c->s0, {}};
     double *din = nullptr, *dout = nullptr;
-    HIPCHK(c, hipMalloc(&din, sizeof(double) * nin));
-    HIPCHK(c, hipMalloc(&dout, sizeof(double) * nout));
+    HIPCHK(c, tmp.get(&din, nin));
+    HIPCHK(c, tmp.get(&dout, nout));
     HIPCHK(c, hipMemcpyAsync(din, in, sizeof(double) * nin, hipMemcpyHostToDevice, c->s0));
     hipError_t e = hipSuccess;
     dispatch_n(c->N, [&](auto Nc) {
@@ -2004,9 +2188,7 @@ static cc_status pack_unpack(cc_ctx *c, const double *in, double *out, int32_t b
     });
     HIPCHK(c, e);
     HIPCHK(c, hipMemcpyAsync(out, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, c->s0));
-    HIPCHK(c, hipStreamSynchronize(c->s0));
-    HIPCHK(c, hipFree(din));
-    HIPCHK(c, hipFree(dout));
+    HIPCHK(c, tmp.release());
     return CC_OK;
 }
 

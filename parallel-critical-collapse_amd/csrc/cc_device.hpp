@@ -153,8 +153,9 @@ constexpr int xswz64_pairs()
     return packed | (k << 24);
 }
 
+// (host and device: the host-side check of the maps calls this same function)
 template <int N, typename R, int NS, int RAD>
-__device__ __forceinline__ int xswz(int i)
+__host__ __device__ __forceinline__ constexpr int xswz(int i)
 {
     if constexpr (N < 64) {
         return i;

@@ -40,3 +40,16 @@ def oracle_for(s):
 
 def ctx_for(s, **kw):
     return cc.Context(cc.Problem.from_shape(s["shape"], **kw))
+
+
+def sample_columns(N, k=64):
+    """k Jacobian columns spread evenly over the Up, psic and fc slots, plus
+    the Delta slot N/2+2: the sample the oracle computes at the large shapes."""
+    n = 3 * N // 4
+    return sorted(set(int(round(v)) for v in np.linspace(0, n - 1, k)) | {N // 2 + 2})
+
+
+def oracle_threads():
+    """Host threads for the oracle's column-parallel Jacobian (at most 16, the
+    host share of one GPU on the test machines)."""
+    return max(1, min(16, os.cpu_count() or 1))

@@ -10,22 +10,11 @@
 #include <cstdio>
 #include <set>
 
+// the kernels' own map (cc_device.hpp), compiled for the host
 template <int N, typename R, int NS, int RAD>
 int host_xswz(int i)
 {
-    if constexpr (N < 64) {
-        return i;
-    } else if constexpr (sizeof(R) == 8) {
-        constexpr int pk = ccd::xswz64_pairs<NS, RAD>(), np = pk >> 24;
-        int x = i;
-        if constexpr (np > 0) x ^= ((i >> ((pk >> 4) & 15)) & 1) << (pk & 15);
-        if constexpr (np > 1) x ^= ((i >> ((pk >> 12) & 15)) & 1) << ((pk >> 8) & 15);
-        if constexpr (np > 2) x ^= ((i >> ((pk >> 20) & 15)) & 1) << ((pk >> 16) & 15);
-        return x;
-    } else {
-        const int g = (i >> 4) & 3;
-        return i ^ (g | (g << 2));
-    }
+    return ccd::xswz<N, R, NS, RAD>(i);
 }
 
 template <int N, int T, typename R, int NS>

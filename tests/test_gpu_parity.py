@@ -239,6 +239,32 @@ def test_newton_step_matches_oracle_c1(c1_reference):
     assert rel(dx, dx_ref) < 0.1
 
 
+@pytest.mark.parametrize("precision", [cc.CC_FP64, cc.CC_MIXED])
+def test_solve_of_the_oracle_jacobian_c1(c1_reference, precision):
+    """cc_solve on the oracle's own FD Jacobian at the 1000-point shape
+    (cond ~ 1e9) with rhs -F0: the system dgesv solves in NewtonSolver.cpp:
+    739-757.  FP64: the oracle's pivots and factors, and its solution within
+    cond(J) eps.  Mixed: the refined solve meets refine_tol (or is re-solved in
+    FP64), and its solution is reported against the oracle's."""
+    s, F0, J_ref, _, _ = c1_reference
+    ref, info_ref, LU_ref, _ = O.dgesv(J_ref, -F0)
+    assert info_ref == 0
+    cond = np.linalg.cond(J_ref)
+    ctx = ctx_for(s, precision=precision)
+    x, info, LU, st = ctx.solve(J_ref, -F0)
+    assert st == 0 and info == 0
+    res = np.linalg.norm(J_ref @ x + F0) / np.linalg.norm(F0)
+    print(f"precision {precision}: cond {cond:.1e}, |x - x_oracle| / |x_oracle| = {rel(x, ref):.2e}, "
+          f"|J x + F0| / |F0| = {res:.2e}, refinement {ctx.refinement()}")
+    if precision == cc.CC_FP64:
+        assert rel(LU, LU_ref) < 1e-10
+        assert rel(x, ref) < 10 * cond * np.finfo(float).eps
+    else:
+        r, resolved = ctx.refinement()
+        assert res <= 1e-6 or resolved
+        assert rel(x, ref) < 1e-2
+
+
 def test_solve_golden_10x10():
     """The system of test/test_lapack.cpp:26-74 (whose written solution 1..10 is
     not its solution; see test_oracle.test_dgesv_golden_10x10): same answer as the
