@@ -101,6 +101,19 @@ cc_status cc_jacobian(cc_ctx *ctx, const double *x, const double *F0, double *J,
  * receives the FP64 factors (cc_refinement reports which). */
 cc_status cc_solve(cc_ctx *ctx, int32_t n, double *A, double *b, int32_t *info);
 
+/* cc_solve of a CC_FP64 context without any allocation, for callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * d_A (n x n row-major) and d_b (n) are device pointers on the current
+ * device: d_A is overwritten by the LU factors, d_b by the solution (left
+ * unchanged when *info > 0, as dgesv); *info (host) follows LAPACK.  Same
+ * kernels and results as cc_solve.  Its scratch (row permutation, panel
+ * swap list, info word: 8960 bytes) is a fixed device array of the library,
+ * one per GPU, so 1 <= n <= 2048 (else CC_BADARG), calls that overlap on one
+ * GPU race, and each GPU keeps the array until the process ends.
+ * Synchronous on the null stream. */
+cc_status cc_dgesv_noalloc(int32_t n, double *d_A, double *d_b, int32_t *info);
+
 /* Outcome of the last CC_MIXED solve of this context (cc_solve,
  * cc_newton_step*, cc_newton_run): *residual = ||J dx - b|| / ||b||
  * after the FP64 refinement of the FP32 solve, *fp64_resolve = 1 when that

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1298,6 +1299,46 @@ cc_status batch_timed(cc_batch *b, double *ms)
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// The FP64 dense solve without allocation (cc_dgesv_noalloc).  Its scratch --
+// the row permutation, the panel swap list and the info word of the
+// factorization -- is the file-scope device array `scratch`: one copy per GPU,
+// held until the process ends, shared by every call on that GPU (overlapping
+// calls on one GPU race).  In host code `scratch` names a host-side stand-in,
+// not device memory: its device address is looked up once per device, with
+// that device current, by hipGetSymbolAddress, and only that address is passed
+// to kernels.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kScratchPerm = 0;                                       // LU_MAXN ints
+constexpr int kScratchSwap = ccd::LU_MAXN;                            // 1 + 2 LU_SWAP_MAX ints
+constexpr int kScratchInfo = kScratchSwap + 1 + 2 * ccd::LU_SWAP_MAX;  // 1 int
+constexpr int kScratchInts = (kScratchInfo + 1 + 63) / 64 * 64;       // 2240 ints = 8960 bytes
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) int32_t scratch[kScratchInts];
+
+// Device address of the current device's copy of `scratch`.
+static hipError_t scratch_address(int32_t **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev] = static_cast<int32_t *>(p);
+    }
+    *out = g_scratch[dev];
+    return hipSuccess;
+}
+
 // ===========================================================================
 // C-ABI
 // ===========================================================================
@@ -2508,6 +2549,33 @@ cc_status cc_batch_autotune(cc_batch *b, const double *const *x, double *times, 
         for (int i = 0; i < std::min<int>(ntimes, (int)t.size()); ++i) times[i] = t[i];
     if (ntried) *ntried = (int32_t)t.size();
     return CC_OK;
+}
+
+cc_status cc_dgesv_noalloc(int32_t n, double *d_A, double *d_b, int32_t *info)
+{
+    if (!d_A || !d_b || !info) return CC_BADARG;
+    if (n < 1 || n > ccd::LU_MAXN) {
+        fprintf(stderr, "cc_dgesv_noalloc: order %d is outside [1, %d], the orders its fixed scratch holds\n", (int)n,
+                ccd::LU_MAXN);
+        return CC_BADARG;
+    }
+    int32_t *sc = nullptr;
+    if (scratch_address(&sc) != hipSuccess) return CC_DEVICE;
+    int32_t *perm = sc + kScratchPerm, *swap = sc + kScratchSwap, *dinfo = sc + kScratchInfo;
+    const hipStream_t s = nullptr;
+    // the info word keeps the last call's value: cleared by a kernel first
+    hipLaunchKernelGGL((ccd::k_fill<int32_t>), dim3(1), dim3(64), 0, s, dinfo, 0, (size_t)1);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = lu_factor_on<double>(d_A, n, perm, swap, dinfo, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL((ccd::k_lu_solve<double, double>), dim3(1), dim3(1024), 0, s, d_A, n, perm, d_b, dinfo);
+        e = hipGetLastError();
+    }
+    int32_t h = 0;
+    if (e == hipSuccess) e = hipMemcpy(&h, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return CC_DEVICE;
+    *info = h;
+    return h != 0 ? CC_SINGULAR : CC_OK;
 }
 
 }  // extern "C"

@@ -314,13 +314,17 @@ __global__ __launch_bounds__(256) void k_lu_gemm(R *A, int n, int k0, int nb)
 }
 
 // Solve with the factors: y = P b (gathered through perm), L y' = y,
-// U x = y'.  One workgroup of 16 waves; b is replaced by x.
+// U x = y'.  One workgroup of 16 waves; b is replaced by x.  info (may be
+// null): when the factorization reported a zero pivot, b is left as it is
+// (dgesv computes no solution then).
 template <typename R, typename RB>
-__global__ __launch_bounds__(1024) void k_lu_solve(const R *A, int n, const int32_t *perm, RB *b)
+__global__ __launch_bounds__(1024) void k_lu_solve(const R *A, int n, const int32_t *perm, RB *b,
+                                                   const int32_t *info = nullptr)
 {
     __shared__ RB bs[LU_MAXN];
     __shared__ R D[64][65];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (info && info[0] != 0) return;
     for (int r = t; r < n; r += 1024) bs[r] = b[perm[r]];
     __syncthreads();
     // unit lower L, 64-row blocks top-down
