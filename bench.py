@@ -404,6 +404,8 @@ def main():
                     help="workload shape 1, 2, 3 or 5; 4 = the batch of uneven problems")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 policy")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="time only the FP64 finite-difference policy and print its record (for profiling runs)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the Newton-to-convergence, Ntau=512 and batch legs")
     ap.add_argument("--no-tune", action="store_true", help="default schedule instead of the autotuner's")
@@ -453,6 +455,10 @@ def main():
     log(f"[bench] workload Ntau={shape.Ntau} X={shape.X} n={shape.n}")
 
     tune = json.loads(args.schedule) if args.schedule else (not args.no_tune)
+    if args.fp64_only:
+        r = fp64_leg(cc, dev, shape, args.steps, args.warmup, tune)
+        print(json.dumps(dict(r, steps=args.steps, warmup=args.warmup)), flush=True)
+        return
     mixed = time_policy(cc, dev, shape, cc.CC_MIXED, args.steps, args.warmup, False, tune)
     fp64 = None if args.no_fp64 else time_policy(cc, dev, shape, cc.CC_FP64, args.steps, args.warmup, False, tune)
     fp64r = None if args.no_fp64 else time_policy(cc, dev, shape, cc.CC_FP64, args.steps, args.warmup, True, tune)
