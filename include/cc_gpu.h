@@ -192,7 +192,7 @@ typedef struct cc_schedule {
     int32_t threads_base;      /* FP64 chains of small launches: residual, mixed base + Delta column */
     int32_t threads_bulk;      /* FP64 chains of full-Jacobian launches */
     int32_t threads_tangent;   /* FP32 tangent-linear chains (CC_MIXED) */
-    int32_t steps_per_launch;  /* IRK steps per chain-kernel launch, 16 .. 8192 */
+    int32_t steps_per_launch;  /* IRK steps per chain-kernel launch, 8 .. 8192 */
     int32_t pipeline;          /* CC_MIXED: tangent chunk k starts as soon as base chunk k is done,
                                   the base running on a second stream */
     int32_t use_graph;         /* Newton steps replay a hipGraph captured on first use */
@@ -207,7 +207,8 @@ enum { CC_TUNE_THREADS = 1, CC_TUNE_STAGES = 2 };
 /* Autotuner.  CC_TUNE_THREADS: times every thread count of each chain
  * kernel on a slice of the integration at x (device work only, a few
  * hundred ms).  CC_TUNE_STAGES: times whole Newton steps at x for
- * steps_per_launch in {64, 128, 256, 512, 1024} and, in CC_MIXED, pipeline on/off.
+ * steps_per_launch in {8, 12, 16, 24, 32, 64} pipelined (tangent-linear policies) and
+ * {64, 128, 256, 512, 1024} on one stream.
  * Installs and returns the fastest schedule.  times (may be NULL) receives
  * up to ntimes candidate timings in ms, in the order tried; *ntried their
  * count. */

@@ -58,7 +58,7 @@ def test_stage_schedule_is_bit_identical():
     F0 = ctx.residual(s["x"])
     ref, cs, st = ctx.jacobian(s["x"], F0)
     assert st == 0 and not cs.any()
-    for pipe, spl in [(0, 512), (1, 64), (0, 100), (1, 1000)]:
+    for pipe, spl in [(0, 512), (1, 64), (0, 100), (1, 1000), (1, 8), (1, 13)]:
         ctx.set_schedule(pipeline=pipe, steps_per_launch=spl)
         J, cs, st = ctx.jacobian(s["x"], F0)
         print("pipeline", pipe, "steps_per_launch", spl, flush=True)
@@ -72,7 +72,7 @@ def test_schedule_validation():
     with pytest.raises(cc.CCError):
         ctx.set_schedule(threads_base=32)
     with pytest.raises(cc.CCError):
-        ctx.set_schedule(steps_per_launch=8)
+        ctx.set_schedule(steps_per_launch=7)
     sc = ctx.schedule()
     assert sc.threads_base == sc.threads_bulk == sc.threads_tangent == 64
 
