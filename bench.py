@@ -49,6 +49,9 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 # by the kernel the bench reports and the steps per launch it was measured at
 # (the record it reads grows with them); anything else reports null.
 MEASURED_TRAFFIC_BYTES = {
+    ("k_tangent<1024,256>", 8): (21.57e6 + 15.98e6),
+    ("k_tangent<1024,256>", 12): (24.18e6 + 16.13e6),
+    ("k_tangent<1024,256>", 16): (26.81e6 + 16.36e6),
     ("k_tangent<1024,256>", 64): (59.59e6 + 24.53e6),
 }
 
