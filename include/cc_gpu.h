@@ -259,11 +259,39 @@ cc_status cc_rhs_eval(cc_ctx *ctx, const double *Y, const double *xs, double *dY
 /* ShootingSolver::integrateToMatchPoint (ShootingSolver.cpp:53-145) over a
  * caller-supplied grid for `batch` independent states: state b starts at
  * grid index start[b] and steps to end[b].  its_total[b] receives the sum of
- * itsReached; status[b] a cc_status.  With nsteps=1 this is
+ * itsReached; status[b] 0, or 1 when a step's fixed point did not converge
+ * (the state then stays at that step's start and fail_idx[b], if fail_idx is
+ * not NULL, receives the grid index the step started from; -1 otherwise).
+ * max_its > 0 replaces MaxIterIRK for this call (the maxIts argument of
+ * ODEStepper::integrate), 0 keeps it.  With one step this is
  * ODEStepper::integrate (ODEStepper.cpp:144-150). */
 cc_status cc_integrate(cc_ctx *ctx, double *Y, const double *grid, int32_t ngrid,
                        const int32_t *start, const int32_t *end, int32_t batch, int32_t fp32,
-                       int64_t *its_total, int32_t *status);
+                       int64_t *its_total, int32_t *status, int32_t max_its, int32_t *fail_idx);
+
+/* SpectralTransformer's operators on `batch` signals of length Ntau (complex
+ * values interleaved re, im):
+ *   CC_SPEC_DIFF    differentiate (SpectralTransformer.cpp:140-158): N complex -> N complex
+ *   CC_SPEC_LAM     lamIntegrate (165-189) with lambda = lambda_re + i lambda_im: N -> N complex
+ *   CC_SPEC_HALVE   halveModes (196-217): N complex -> N/2 complex
+ *   CC_SPEC_DOUBLE  doubleModes (224-252): N/2 complex -> N complex
+ *   CC_SPEC_INHOM   solveInhom (264-292): in = [batch][f (N real), g (N real)] -> out = [batch][N real]
+ * The wavenumber is 2 pi / period (period > 0; HALVE and DOUBLE ignore it). */
+typedef enum cc_spectral_op {
+    CC_SPEC_DIFF = 0,
+    CC_SPEC_LAM = 1,
+    CC_SPEC_HALVE = 2,
+    CC_SPEC_DOUBLE = 3,
+    CC_SPEC_INHOM = 4
+} cc_spectral_op;
+cc_status cc_spectral(cc_ctx *ctx, int32_t op, const double *in, double *out, int32_t batch, double period,
+                      double lambda_re, double lambda_im);
+
+/* InitialConditionGenerator::StateVectorToFields (ICG.cpp:891-932) for
+ * `batch` states Y (M complex each) at abscissae xs: fields[b][4][Ntau] =
+ * A = sqrt(1/IA2), U, V, F, the Debug snapshot of ShootingSolver.cpp:85-97.
+ * Delta of each state is Re Y[2]. */
+cc_status cc_state_fields(cc_ctx *ctx, const double *Y, const double *xs, double *fields, int32_t batch);
 
 /* ICG::packSpectralFields (ICG.cpp:678-711) for `batch` field sets:
  * fields = [batch][3][Ntau] real tau-series Up, psic, fc -> x = [batch][n].

@@ -74,6 +74,7 @@ class Schedule(C.Structure):
 
 
 CC_TUNE_THREADS, CC_TUNE_STAGES = 1, 2
+CC_SPEC_DIFF, CC_SPEC_LAM, CC_SPEC_HALVE, CC_SPEC_DOUBLE, CC_SPEC_INHOM = range(5)
 
 
 def declared_symbols():
@@ -115,7 +116,9 @@ def load():
     L.cc_synchronize.argtypes = [_vp]
     L.cc_fft_batched.argtypes = [_vp, _dp, _dp, C.c_int32, C.c_int32, C.c_int32]
     L.cc_rhs_eval.argtypes = [_vp, _dp, _dp, _dp, C.c_int32, C.c_int32]
-    L.cc_integrate.argtypes = [_vp, _dp, _dp, C.c_int32, _ip, _ip, C.c_int32, C.c_int32, _lp, _ip]
+    L.cc_integrate.argtypes = [_vp, _dp, _dp, C.c_int32, _ip, _ip, C.c_int32, C.c_int32, _lp, _ip, C.c_int32, _ip]
+    L.cc_spectral.argtypes = [_vp, C.c_int32, _dp, _dp, C.c_int32, C.c_double, C.c_double, C.c_double]
+    L.cc_state_fields.argtypes = [_vp, _dp, _dp, _dp, C.c_int32]
     L.cc_expansions.argtypes = [_vp, _dp, _dp, _dp]
     L.cc_profile.argtypes = [_vp, C.c_int32]
     L.cc_chain_timing.argtypes = [_vp, _dp, _ip, _lp, _lp]
@@ -337,7 +340,9 @@ class Context:
                                        Y.shape[0], int(fp32)), "cc_rhs_eval")
         return dY
 
-    def integrate(self, Y, grid, start, end, fp32=False):
+    def integrate(self, Y, grid, start, end, fp32=False, max_its=0, with_fail_idx=False):
+        """Integrate states Y[b] from grid[start[b]] to grid[end[b]]; returns (Y, its,
+        status), and the failing steps' start indices (-1: converged) with_fail_idx."""
         Y = np.array(np.atleast_2d(Y), np.complex128)
         grid = _arr(grid)
         start = _arr(np.atleast_1d(start), np.int32)
@@ -347,11 +352,49 @@ class Context:
                              f"got {Y.shape}, {start.shape}, {end.shape}")
         its = np.zeros(Y.shape[0], np.int64)
         status = np.zeros(Y.shape[0], np.int32)
+        fail = np.zeros(Y.shape[0], np.int32)
         self._check(load().cc_integrate(self.h, Y.ctypes.data_as(_dp), _d(grid), grid.size,
                                         start.ctypes.data_as(_ip), end.ctypes.data_as(_ip), Y.shape[0],
-                                        int(fp32), its.ctypes.data_as(_lp), status.ctypes.data_as(_ip)),
+                                        int(fp32), its.ctypes.data_as(_lp), status.ctypes.data_as(_ip),
+                                        int(max_its), fail.ctypes.data_as(_ip)),
                     "cc_integrate")
-        return Y, its, status
+        return (Y, its, status, fail) if with_fail_idx else (Y, its, status)
+
+    def spectral(self, op, X, period=1.0, lam=0.0):
+        """SpectralTransformer's operators on a batch (CC_SPEC_*): DIFF, LAM and
+        HALVE take N complex values per signal, DOUBLE N/2 complex, INHOM a pair
+        (f, g) of N real values ([batch][2][N])."""
+        N = self.N
+        shapes = {CC_SPEC_DIFF: ((N,), np.complex128, N, np.complex128),
+                  CC_SPEC_LAM: ((N,), np.complex128, N, np.complex128),
+                  CC_SPEC_HALVE: ((N,), np.complex128, N // 2, np.complex128),
+                  CC_SPEC_DOUBLE: ((N // 2,), np.complex128, N, np.complex128),
+                  CC_SPEC_INHOM: ((2, N), np.float64, N, np.float64)}
+        if op not in shapes:
+            raise ValueError(f"spectral: unknown operator {op}")
+        ishape, idt, nout, odt = shapes[op]
+        X = np.asarray(X)
+        one = X.ndim == len(ishape)
+        X = _arr(X[None] if one else X, idt)
+        if X.shape[1:] != ishape:
+            raise ValueError(f"spectral: expected signals of shape {ishape}, got {X.shape}")
+        out = np.zeros((X.shape[0], nout), odt)
+        lam = complex(lam)
+        self._check(load().cc_spectral(self.h, int(op), X.ctypes.data_as(_dp), out.ctypes.data_as(_dp),
+                                       X.shape[0], float(period), lam.real, lam.imag), "cc_spectral")
+        return out[0] if one else out
+
+    def state_fields(self, Y, xs):
+        """States (M complex each) at abscissae xs -> [batch][4][N] = A, U, V, F."""
+        Y = _arr(np.atleast_2d(Y), np.complex128)
+        xs = _arr(np.atleast_1d(xs))
+        if Y.ndim != 2 or Y.shape[1] != self.M or xs.shape != (Y.shape[0],):
+            raise ValueError(f"state_fields: expected states of {self.M} modes and one abscissa each, "
+                             f"got {Y.shape}, {xs.shape}")
+        f = np.zeros((Y.shape[0], 4, self.N))
+        self._check(load().cc_state_fields(self.h, Y.ctypes.data_as(_dp), _d(xs), _d(f), Y.shape[0]),
+                    "cc_state_fields")
+        return f
 
     def pack(self, Up, psic, fc):
         """Three real tau-series (N each, or [batch, N]) -> Newton vector(s)."""

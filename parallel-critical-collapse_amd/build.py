@@ -3,7 +3,9 @@
     python parallel-critical-collapse_amd/build.py [-v] [-j JOBS]
 
 The library holds the HIP kernels and the C-ABI of include/cc_gpu.h; the
-executable cc_newton is the C++ NewtonSolver driver over it (host/).  The
+executable cc_newton is the C++ NewtonSolver driver over it, cc_layers the
+driver of the SpectralTransformer / ODEStepper / ShootingSolver classes
+(host/).  The
 length-N kernels are split over one translation unit per supported Ntau
 (csrc/cc_ops_<N>.hip) so that they compile in parallel; objects go to
 _build/ next to this file, the library next to this file.
@@ -27,6 +29,9 @@ HEADERS = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith
 
 HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("newton.cpp", "suite.cpp", "cc_newton.cpp")]
 CLI = os.path.join(HERE, "cc_newton")
+LAYER_SOURCES = [os.path.join(HERE, "host", f) for f in ("newton.cpp", "layers.cpp", "cc_layers.cpp")]
+LAYER_CLI = os.path.join(HERE, "cc_layers")
+HOST_HEADERS = [os.path.join(HERE, "host", f) for f in ("newton.hpp", "suite.hpp", "layers.hpp", "json.hpp")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
@@ -65,16 +70,17 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         if out.strip():
             print(out)
     os.replace(tmp, LIB)
-    # host-side NewtonSolver driver (plain C++ over the C-ABI)
-    host = [CXX, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), *HOST_SOURCES,
-            "-L", HERE, "-lcc_gpu", "-L/opt/rocm/lib", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib",
-            "-o", CLI + ".tmp"]
-    out = _run(host)
-    if verbose:
-        print(" ".join(host))
-        if out.strip():
-            print(out)
-    os.replace(CLI + ".tmp", CLI)
+    # host-side drivers (plain C++ over the C-ABI): NewtonSolver and the lower layers
+    for srcs, exe in ((HOST_SOURCES, CLI), (LAYER_SOURCES, LAYER_CLI)):
+        host = [CXX, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), *srcs,
+                "-L", HERE, "-lcc_gpu", "-L/opt/rocm/lib", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib",
+                "-o", exe + ".tmp"]
+        out = _run(host)
+        if verbose:
+            print(" ".join(host))
+            if out.strip():
+                print(out)
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 
@@ -82,7 +88,8 @@ def stale() -> bool:
     if not os.path.exists(LIB):
         return True
     t = os.path.getmtime(LIB)
-    return any(os.path.getmtime(f) > t for f in SOURCES + HEADERS + HOST_SOURCES) or not os.path.exists(CLI)
+    inputs = SOURCES + HEADERS + HOST_SOURCES + LAYER_SOURCES + HOST_HEADERS
+    return any(os.path.getmtime(f) > t for f in inputs) or not all(os.path.exists(e) for e in (CLI, LAYER_CLI))
 
 
 if __name__ == "__main__":

@@ -1951,6 +1951,35 @@ cc_status cc_fft_batched(cc_ctx *c, const double *in, double *out, int32_t batch
     return CC_OK;
 }
 
+cc_status cc_spectral(cc_ctx *c, int32_t op, const double *in, double *out, int32_t batch, double period,
+                      double lambda_re, double lambda_im)
+{
+    if (!c || !in || !out || batch < 1 || op < CC_SPEC_DIFF || op > CC_SPEC_INHOM) return CC_BADARG;
+    const bool uses_period = op == CC_SPEC_DIFF || op == CC_SPEC_LAM || op == CC_SPEC_INHOM;
+    if (uses_period && !(period > 0.0 && std::isfinite(period))) {
+        c->err = "cc_spectral: the period must be positive and finite";
+        return CC_BADARG;
+    }
+    const int N = c->N;
+    // doubles per signal in and out
+    const size_t din_n = op == CC_SPEC_DOUBLE ? (size_t)N : 2 * (size_t)N;
+    const size_t dout_n = op == CC_SPEC_HALVE || op == CC_SPEC_INHOM ? (size_t)N : 2 * (size_t)N;
+    CallBuffers tmp{c->s0, {}};
+    double *din = nullptr, *dout = nullptr;
+    HIPCHK(c, tmp.get(&din, din_n * batch));
+    HIPCHK(c, tmp.get(&dout, dout_n * batch));
+    HIPCHK(c, hipMemcpyAsync(din, in, sizeof(double) * din_n * batch, hipMemcpyHostToDevice, c->s0));
+    const double k0 = uses_period ? 2.0 * M_PI / period : 0.0;     // SpectralTransformer.cpp:23, 142, 167
+    hipError_t e = hipSuccess;
+    dispatch_n(N, [&](auto Nc) {
+        e = Ops<decltype(Nc)::value>::spectral(c->s0, op, batch, din, dout, c->tw64, k0, lambda_re, lambda_im);
+    });
+    HIPCHK(c, e);
+    HIPCHK(c, hipMemcpyAsync(out, dout, sizeof(double) * dout_n * batch, hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, tmp.release());
+    return CC_OK;
+}
+
 cc_status cc_rhs_eval(cc_ctx *c, const double *Y, const double *xs, double *dY, int32_t batch, int32_t fp32)
 {
     if (!c || !Y || !xs || !dY || batch < 1) return CC_BADARG;
@@ -2002,6 +2031,57 @@ cc_status cc_rhs_eval(cc_ctx *c, const double *Y, const double *xs, double *dY, 
     return CC_OK;
 }
 
+// Fields of FP64 states on the device, paired as k_rhs_pairs evaluates them
+// (snap: 2 npairs slots of M complex; xs, delta: 2 npairs host values, the two
+// members of a pair sharing delta[2 pr]): output k (k < count) is slot
+// k * slot_stride, fields[k][4][N] = A = sqrt(1/IA2), U, V, F
+// (ICG::StateVectorToFields, ICG.cpp:891-932; A as in ShootingSolver.cpp:91).
+static cc_status fields_of_states(cc_ctx *c, const cx<double> *snap, int npairs, const std::vector<double> &xs,
+                                  const std::vector<double> &delta, int count, int slot_stride, double *fields)
+{
+    const int N = c->N, M = c->M;
+    const size_t step = (size_t)rec_step_values(N), stage = step / 2;
+    CallBuffers tmp{c->s0, {}};
+    cx<double> *dY = nullptr;
+    double *sx = nullptr, *sd = nullptr, *rec = nullptr;
+    HIPCHK(c, tmp.get(&dY, 2 * (size_t)npairs * M));
+    HIPCHK(c, tmp.get(&sx, 2 * (size_t)npairs));
+    HIPCHK(c, tmp.get(&sd, 2 * (size_t)npairs));
+    HIPCHK(c, tmp.get(&rec, step * npairs));
+    HIPCHK(c, hipMemcpyAsync(sx, xs.data(), sizeof(double) * 2 * npairs, hipMemcpyHostToDevice, c->s0));
+    HIPCHK(c, hipMemcpyAsync(sd, delta.data(), sizeof(double) * 2 * npairs, hipMemcpyHostToDevice, c->s0));
+    int oU = 0, oV = 0, oF = 0, oR = 0;          // record offsets of U, V, F (j < M) and r = 1/IA2
+    hipError_t e = hipSuccess;
+    dispatch_n(N, [&](auto Nc) {
+        using RC = ccd::Rec<decltype(Nc)::value>;
+        oU = RC::U;
+        oV = RC::V;
+        oF = RC::F;
+        oR = RC::R;
+        e = Ops<decltype(Nc)::value>::rhs_pairs(c->s0, false, npairs, snap, sx, sd, dY, c->tw64, c->invm64,
+                                                 c->p.Dim, rec);
+    });
+    HIPCHK(c, e);
+    std::vector<double> rh(step * npairs);
+    HIPCHK(c, hipMemcpyAsync(rh.data(), rec, sizeof(double) * rh.size(), hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, tmp.release());                     // synchronizes the stream
+    for (int k = 0; k < count; ++k) {
+        const int s = k * slot_stride;
+        const double *r = rh.data() + (size_t)(s / 2) * step + (size_t)(s % 2) * stage;
+        double *A = fields + (size_t)k * 4 * N, *U = A + N, *V = U + N, *F = V + N;
+        for (int j = 0; j < M; ++j) {       // second half by the symmetries of ICG.cpp:819-824
+            U[j] = r[oU + j];
+            V[j] = r[oV + j];
+            F[j] = r[oF + j];
+            U[j + M] = -U[j];
+            V[j + M] = -V[j];
+            F[j + M] = F[j];
+        }
+        for (int j = 0; j < N; ++j) A[j] = std::sqrt(r[oR + j]);    // ShootingSolver.cpp:91
+    }
+    return CC_OK;
+}
+
 cc_status cc_shoot_fields(cc_ctx *c, const double *x, int32_t every, int32_t capacity, int32_t *count, double *xs,
                           double *fields)
 {
@@ -2022,24 +2102,13 @@ cc_status cc_shoot_fields(cc_ctx *c, const double *x, int32_t every, int32_t cap
     HIPCHK(c, hipMemcpy(g.data(), c->grid, sizeof(double) * g.size(), hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpyAsync(c->x, x, sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
     const int npairs = (total + 1) / 2;
-    const size_t step = (size_t)rec_step_values(N), stage = step / 2;
     std::vector<double> sxh(2 * (size_t)npairs), sdh(2 * (size_t)npairs, x[N / 2 + 2]);
-    cx<double> *snap = nullptr, *dY = nullptr;
-    double *sx = nullptr, *sd = nullptr, *rec = nullptr;
-    hipError_t e = hipMalloc(&snap, sizeof(cx<double>) * 2 * npairs * M);
-    if (e == hipSuccess) e = hipMalloc(&dY, sizeof(cx<double>) * 2 * npairs * M);
-    if (e == hipSuccess) e = hipMalloc(&sx, sizeof(double) * 2 * npairs);
-    if (e == hipSuccess) e = hipMalloc(&sd, sizeof(double) * 2 * npairs);
-    if (e == hipSuccess) e = hipMalloc(&rec, sizeof(double) * step * npairs);
-    auto release = [&]() {
-        for (void *q : {(void *)snap, (void *)dY, (void *)sx, (void *)sd, (void *)rec}) (void)hipFree(q);
-    };
-    if (e != hipSuccess) {
-        release();
-        HIPCHK(c, e);
-    }
+    CallBuffers tmp{c->s0, {}};
+    cx<double> *snap = nullptr;
+    HIPCHK(c, tmp.get(&snap, 2 * (size_t)npairs * M));
     const Counts k{1, 1, 2, 1};                 // the base column: chain 0 left, chain 1 right
     cc_status st = CC_OK;
+    hipError_t e = hipSuccess;
     dispatch_n(N, [&](auto Nc) {
         using O = Ops<decltype(Nc)::value>;
         st = launch_prologue<decltype(Nc)::value>(c, c->base, k, c->x, 0.0, c->s0);
@@ -2074,56 +2143,49 @@ cc_status cc_shoot_fields(cc_ctx *c, const double *x, int32_t every, int32_t cap
                                hipMemcpyDeviceToDevice, c->s0);
             sxh[total] = sxh[total - 1];
         }
-        if (st == CC_OK && e == hipSuccess)
-            e = hipMemcpyAsync(sx, sxh.data(), sizeof(double) * sxh.size(), hipMemcpyHostToDevice, c->s0);
-        if (st == CC_OK && e == hipSuccess)
-            e = hipMemcpyAsync(sd, sdh.data(), sizeof(double) * sdh.size(), hipMemcpyHostToDevice, c->s0);
-        if (st == CC_OK && e == hipSuccess)
-            e = O::rhs_pairs(c->s0, false, npairs, snap, sx, sd, dY, c->tw64, c->invm64, c->p.Dim, rec);
     });
-    std::vector<double> rh(step * npairs);
-    int32_t chst[2] = {0, 0};
-    if (st == CC_OK && e == hipSuccess)
-        e = hipMemcpyAsync(rh.data(), rec, sizeof(double) * rh.size(), hipMemcpyDeviceToHost, c->s0);
-    if (st == CC_OK && e == hipSuccess)
-        e = hipMemcpyAsync(chst, c->base.status, sizeof(chst), hipMemcpyDeviceToHost, c->s0);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->s0);
-    release();
     CCCHK(st);
     HIPCHK(c, e);
+    int32_t chst[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(chst, c->base.status, sizeof(chst), hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
     if (chst[0] || chst[1]) {
         c->err = "IRK fixed-point iteration did not converge in the field-snapshot shoot";
         return CC_IRK_NOCONV;
     }
-    int oU = 0, oV = 0, oF = 0, oR = 0;          // record offsets of U, V, F (j < M) and r = 1/IA2
-    dispatch_n(N, [&](auto Nc) {
-        using RC = ccd::Rec<decltype(Nc)::value>;
-        oU = RC::U;
-        oV = RC::V;
-        oF = RC::F;
-        oR = RC::R;
-    });
-    for (int s = 0; s < total; ++s) {
-        const double *r = rh.data() + (size_t)(s / 2) * step + (size_t)(s % 2) * stage;
-        double *A = fields + (size_t)s * 4 * N, *U = A + N, *V = U + N, *F = V + N;
-        for (int j = 0; j < M; ++j) {       // second half by the symmetries of ICG.cpp:819-824
-            U[j] = r[oU + j];
-            V[j] = r[oV + j];
-            F[j] = r[oF + j];
-            U[j + M] = -U[j];
-            V[j + M] = -V[j];
-            F[j + M] = F[j];
-        }
-        for (int j = 0; j < N; ++j) A[j] = std::sqrt(r[oR + j]);    // ShootingSolver.cpp:91
-        xs[s] = sxh[s];
+    CCCHK(fields_of_states(c, snap, npairs, sxh, sdh, total, 1, fields));
+    for (int s = 0; s < total; ++s) xs[s] = sxh[s];
+    HIPCHK(c, tmp.release());
+    return CC_OK;
+}
+
+cc_status cc_state_fields(cc_ctx *c, const double *Y, const double *xs, double *fields, int32_t batch)
+{
+    if (!c || !Y || !xs || !fields || batch < 1) return CC_BADARG;
+    const int M = c->M;
+    // each state is both members of its own pair: the members of a pair share Delta
+    std::vector<double> xh(2 * (size_t)batch), dh(2 * (size_t)batch);
+    for (int b = 0; b < batch; ++b) {
+        xh[2 * b] = xh[2 * b + 1] = xs[b];
+        dh[2 * b] = dh[2 * b + 1] = Y[(size_t)b * 2 * M + 4];        // Delta = Re Y[2]
     }
+    CallBuffers tmp{c->s0, {}};
+    cx<double> *snap = nullptr;
+    HIPCHK(c, tmp.get(&snap, 2 * (size_t)batch * M));
+    for (int b = 0; b < batch; ++b)
+        for (int m = 0; m < 2; ++m)
+            HIPCHK(c, hipMemcpyAsync(snap + (size_t)(2 * b + m) * M, Y + (size_t)b * 2 * M, sizeof(cx<double>) * M,
+                                     hipMemcpyHostToDevice, c->s0));
+    CCCHK(fields_of_states(c, snap, batch, xh, dh, batch, 2, fields));
+    HIPCHK(c, tmp.release());
     return CC_OK;
 }
 
 cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, const int32_t *start,
-                       const int32_t *end, int32_t batch, int32_t fp32, int64_t *its_total, int32_t *status)
+                       const int32_t *end, int32_t batch, int32_t fp32, int64_t *its_total, int32_t *status,
+                       int32_t max_its, int32_t *fail_idx)
 {
-    if (!c || !Y || !grid || ngrid < 2 || !start || !end || batch < 1) return CC_BADARG;
+    if (!c || !Y || !grid || ngrid < 2 || !start || !end || batch < 1 || max_its < 0) return CC_BADARG;
     if (fp32 && c->stages != 2) {
         c->err = "FP32 chains integrate the two-stage scheme only";
         return CC_BADARG;
@@ -2162,6 +2224,7 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
     HIPCHK(c, hipMemcpyAsync(dns, ns.data(), sizeof(int32_t) * batch, hipMemcpyHostToDevice, c->s0));
     HIPCHK(c, hipMemcpyAsync(ddr, dr.data(), sizeof(int32_t) * batch, hipMemcpyHostToDevice, c->s0));
     CCCHK(fill<int32_t>(c, dstat, 0, batch, c->s0));
+    CCCHK(fill<int32_t>(c, dfail, -1, batch, c->s0));
     CCCHK(fill<int64_t>(c, dits, 0, batch, c->s0));
     const unsigned g = (unsigned)((2 * nY + 255) / 256);
     hipError_t e = hipSuccess;
@@ -2181,7 +2244,7 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
             a.its = dits;
             a.tw = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? (const void *)c->tw64 : (const void *)c->tw32);
             a.invm = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c->invm64 : (const void *)c->invm32);
-            a.maxIts = c->p.MaxIterIRK;
+            a.maxIts = max_its > 0 ? max_its : c->p.MaxIterIRK;
             a.Dim = c->p.Dim;
             a.precision = sizeof(R) == 8 ? c->p.PrecisionIRK : c->p.PrecisionIRK32;
             set_tableau(a, c);
@@ -2205,13 +2268,16 @@ cc_status cc_integrate(cc_ctx *c, double *Y, const double *grid, int32_t ngrid, 
     HIPCHK(c, e);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(Y, dtmp, sizeof(cx<double>) * nY, hipMemcpyDeviceToHost, c->s0));
-    std::vector<int32_t> stv(batch);
+    std::vector<int32_t> stv(batch), fiv(batch);
     std::vector<int64_t> itv(batch);
     HIPCHK(c, hipMemcpyAsync(stv.data(), dstat, sizeof(int32_t) * batch, hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipMemcpyAsync(fiv.data(), dfail, sizeof(int32_t) * batch, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipMemcpyAsync(itv.data(), dits, sizeof(int64_t) * batch, hipMemcpyDeviceToHost, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));
     if (status)
         for (int b = 0; b < batch; ++b) status[b] = stv[b];
+    if (fail_idx)
+        for (int b = 0; b < batch; ++b) fail_idx[b] = fiv[b];
     if (its_total)
         for (int b = 0; b < batch; ++b) its_total[b] = itv[b];
     HIPCHK(c, tmp.release());

@@ -167,4 +167,62 @@ __global__ __launch_bounds__(T) void k_fft(const cx<double> *in, cx<double> *out
     }
 }
 
+// The spectral operators of SpectralTransformer on a batch, one signal per
+// workgroup (unit entry point).  op (SPEC_*): DIFF, LAM: N complex -> N complex
+// (SpectralTransformer.cpp:140-189, wavenumber k0 = 2 pi / period, lambda =
+// lr + i li); HALVE: N complex -> N/2 (196-217); DOUBLE: N/2 complex -> N
+// (224-252); INHOM: f, g (N real each, one after the other) -> x (N real),
+// the periodic solution of x' + f x + g = 0 (264-292).
+enum SpecOp { SPEC_DIFF = 0, SPEC_LAM = 1, SPEC_HALVE = 2, SPEC_DOUBLE = 3, SPEC_INHOM = 4 };
+
+template <int N, int T>
+__global__ __launch_bounds__(T) void k_spectral(int op, const double *in, double *out, const cx<double> *tw,
+                                                double k0, double lr, double li)
+{
+    constexpr int P = N / T, H = P / 2;
+    __shared__ cx<double> xb[N];
+    __shared__ double bc[4];
+    const int t = threadIdx.x;
+    const size_t b = blockIdx.x;
+    Spec<N, T> S{t, xb, bc, tw, k0};
+    if (op == SPEC_INHOM) {
+        const double *f = in + b * 2 * N, *g = f + N;
+        double fv[P], gv[P], x[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            fv[p] = f[t + T * p];
+            gv[p] = g[t + T * p];
+        }
+        S.solve_inhom(fv, gv, x);
+#pragma unroll
+        for (int p = 0; p < P; ++p) out[b * N + t + T * p] = x[p];
+        return;
+    }
+    cx<double> z[P];
+    if (op == SPEC_DOUBLE) {
+        const cx<double> *Y = reinterpret_cast<const cx<double> *>(in) + b * (N / 2);
+        cx<double> Yh[H];
+#pragma unroll
+        for (int h = 0; h < H; ++h) Yh[h] = Y[t + T * h];
+        S.dbl(Yh, z);
+    } else {
+        const cx<double> *X = reinterpret_cast<const cx<double> *>(in) + b * N;
+#pragma unroll
+        for (int p = 0; p < P; ++p) z[p] = X[t + T * p];
+    }
+    if (op == SPEC_HALVE) {
+        cx<double> Yh[H];
+        S.halve(z, Yh);
+        cx<double> *Y = reinterpret_cast<cx<double> *>(out) + b * (N / 2);
+#pragma unroll
+        for (int h = 0; h < H; ++h) Y[t + T * h] = Yh[h];
+        return;
+    }
+    if (op == SPEC_DIFF) S.diff(z);
+    else if (op == SPEC_LAM) S.lam(z, lr, li);
+    cx<double> *Z = reinterpret_cast<cx<double> *>(out) + b * N;
+#pragma unroll
+    for (int p = 0; p < P; ++p) Z[t + T * p] = z[p];
+}
+
 }  // namespace ccd

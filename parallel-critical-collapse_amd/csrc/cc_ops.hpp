@@ -51,12 +51,15 @@ struct Ops {
     // ICG pack / unpack of a batch of Newton vectors (fields: [batch][3][N], Z: [batch][n])
     static hipError_t pack(hipStream_t s, int batch, const double *fields, const cx<double> *tw, double *Z);
     static hipError_t unpack(hipStream_t s, int batch, const double *Z, const cx<double> *tw, double *fields);
-    // unit entry points: batched transform and paired RHS evaluation
+    // unit entry points: batched transform, paired RHS evaluation, spectral operators
     static hipError_t fft(hipStream_t s, bool fp32, int sign, int batch, const cx<double> *in, cx<double> *out,
                           const void *tw);
     static hipError_t rhs_pairs(hipStream_t s, bool fp32, int npairs, const void *Y, const double *xs,
                                 const double *delta, void *dY, const void *tw, const void *invm, double Dim,
                                 void *rec = nullptr);
+    // SpectralTransformer's operators (k_spectral, op a SpecOp) on `batch` signals
+    static hipError_t spectral(hipStream_t s, int op, int batch, const double *in, double *out,
+                               const cx<double> *tw, double k0, double lr, double li);
 };
 
 extern template struct Ops<128>;

@@ -218,4 +218,13 @@ hipError_t Ops<N>::rhs_pairs(hipStream_t s, bool fp32, int npairs, const void *Y
     return hipGetLastError();
 }
 
+template <int N>
+hipError_t Ops<N>::spectral(hipStream_t s, int op, int batch, const double *in, double *out, const cx<double> *tw,
+                            double k0, double lr, double li)
+{
+    hipLaunchKernelGGL((k_spectral<N, kThreads>), dim3((unsigned)batch), dim3(kThreads), 0, s, op, in, out, tw, k0,
+                       lr, li);
+    return hipGetLastError();
+}
+
 }  // namespace ccd

@@ -193,7 +193,7 @@ void NewtonSolver::write_jacobian_csv(size_t iteration) const
 }
 
 // common.cpp:201-215: elements i % n == 0 and the last one
-static std::vector<double> every_nth_element(const double *a, size_t size, size_t n)
+std::vector<double> every_nth_element(const double *a, size_t size, size_t n)
 {
     std::vector<double> out;
     for (size_t i = 0; i < size; ++i)

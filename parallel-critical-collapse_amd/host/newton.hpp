@@ -55,6 +55,9 @@ struct SolverError : std::runtime_error {
     SolverError(cc_status s, const std::string &m) : std::runtime_error(m), status(s) {}
 };
 
+// common.cpp:201-215: the elements a[i] with i % n == 0, and the last one
+std::vector<double> every_nth_element(const double *a, size_t size, size_t n);
+
 class NewtonSolver {
 public:
     NewtonSolver(const SimulationConfig &config, const SolverOptions &opt = {}, bool benchmark = false);

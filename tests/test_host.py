@@ -144,3 +144,33 @@ def test_suite_first_dimension_needs_data(tmp_path):
     r = run(["-m", "-i", str(p)])
     assert r.returncode != 0
     assert "No converged solution as initial conditions available for 4.000!" in r.stderr
+
+
+LAYERS = os.path.join(ROOT, "parallel-critical-collapse_amd", "cc_layers")
+
+
+def run_layers(tmp_path, text):
+    p, q = tmp_path / "req.json", tmp_path / "out.json"
+    p.write_text(text)
+    r = subprocess.run([LAYERS, str(p), str(q)], capture_output=True, text=True, timeout=60)
+    return r, (json.loads(q.read_text())["results"] if r.returncode == 0 else None)
+
+
+def test_layers_driver_rejects_bad_requests(tmp_path):
+    """cc_layers (the SpectralTransformer / ODEStepper / ShootingSolver driver)
+    checks its request before any device work: a malformed file, a missing
+    call list or an IRK scheme other than 1, 2, 3 end the run; a transform
+    length the library lacks and an unknown method fail their call only."""
+    for text, msg in (("{", "json"), (json.dumps({"Ntau": 128}), "calls"),
+                      (json.dumps({"Ntau": 128, "SchemeIRK": 4, "calls": []}), "Wrong IRK Scheme")):
+        r, _ = run_layers(tmp_path, text)
+        assert r.returncode != 0 and msg in r.stderr, (text, r.stderr)
+    calls = [{"method": "differentiate", "in": [1.0, 0.0]}, {"method": "nope"}]
+    r, res = run_layers(tmp_path, json.dumps({"Ntau": 100, "calls": calls}))
+    assert r.returncode == 0
+    assert res[0]["status"] == 4 and "Ntau must be 128, 256, 512, 1024 or 2048" in res[0]["error"]
+    assert res[1]["status"] == 4 and "unknown method 'nope'" in res[1]["error"]
+    r, _ = run_layers(tmp_path, "{}")
+    assert r.returncode != 0
+    r = subprocess.run([LAYERS], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "usage" in r.stderr
