@@ -35,6 +35,7 @@ import math
 import os
 import sys
 import time
+from dataclasses import replace
 
 import numpy as np
 
@@ -80,14 +81,18 @@ def hbm_bytes_step(shape):
     return 8.0 * (n * (n + 1) + n * n + shape.X + 3 * n)
 
 
-def time_policy(cc, dev, shape, precision, steps, warmup, reuse, tune):
+def time_policy(cc, dev, shape, precision, steps, warmup, reuse, tune, eps=None):
     """Autotune (outside the timed region), then time `steps` Newton steps on
-    device-resident x, F0, dx.  Also returns the Jacobian at x."""
+    device-resident x, F0, dx.  Also returns the Jacobian at x.  eps: an FD
+    step other than the shape's EpsNewton."""
     n = shape.n
-    ctx = cc.Context(cc.Problem.from_shape(shape, precision=precision, reuse_sides=int(reuse)))
     x = importlib.import_module("parallel-critical-collapse_amd.workload").initial_x(shape)
-    name = ("mixed" if precision == cc.CC_MIXED else "fp64") + ("+reuse" if reuse else "")
-    tangent = precision != cc.CC_FP64
+    if eps is not None:
+        shape = replace(shape, EpsNewton=eps)
+    ctx = cc.Context(cc.Problem.from_shape(shape, precision=precision, reuse_sides=int(reuse)))
+    name = {cc.CC_MIXED: "mixed", cc.CC_FP64: "fp64", cc.CC_FP64_TANGENT: "fp64_tangent",
+            cc.CC_FP32_FD: "fp32_fd"}[precision] + ("+reuse" if reuse else "")
+    tangent = precision in (cc.CC_MIXED, cc.CC_FP64_TANGENT)
     tuned = None
     if isinstance(tune, dict):
         ctx.set_schedule(**tune)
@@ -127,7 +132,7 @@ def time_policy(cc, dev, shape, precision, steps, warmup, reuse, tune):
             base = np.array(ctx.base_timing()) * steps
     F0, dx = d_F0.download(), d_dx.download()
     J = ctx.last_jacobian()
-    refine = ctx.refinement() if precision == cc.CC_MIXED else None
+    refine = ctx.refinement() if precision in (cc.CC_MIXED, cc.CC_FP32_FD) else None
     ctx.close()
     return {
         "wall_s": wall, "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
@@ -240,11 +245,13 @@ def newton_study(cc, shape, policies, tol, max_its, tune, save_failures=None, sc
     policies.  schedules: policy name -> schedule fields to install instead of
     autotuning."""
     W = importlib.import_module("parallel-critical-collapse_amd.workload")
-    names = {"fp64": cc.CC_FP64, "mixed": cc.CC_MIXED, "fp64_tangent": cc.CC_FP64_TANGENT}
+    # fp32_fd: FP32 shoots at FD step 1e-3 (the reference's 1e-8 is below their rounding)
+    names = {"fp64": cc.CC_FP64, "mixed": cc.CC_MIXED, "fp64_tangent": cc.CC_FP64_TANGENT, "fp32_fd": cc.CC_FP32_FD}
     dslot = shape.Ntau // 2 + 2
     out = {}
     for name in policies:
-        ctx = cc.Context(cc.Problem.from_shape(shape, precision=names[name]))
+        sh = replace(shape, EpsNewton=1e-3) if name == "fp32_fd" else shape
+        ctx = cc.Context(cc.Problem.from_shape(sh, precision=names[name]))
         x = W.initial_x(shape)
         if schedules and name in schedules:
             ctx.set_schedule(**schedules[name])
@@ -373,6 +380,31 @@ def fp64_leg(cc, dev, shape, steps, warmup, tune):
             "kbar": round(r["kbar"], 3), "schedule": r["schedule"]}
 
 
+def fp32_fd_leg(cc, dev, shape, steps, tune, ocols, dxo, mixed_steps_per_s):
+    """The reference's finite differences with every shoot in FP32
+    (CC_FP32_FD) at this shape, at FD steps h = 1e-3 and 1e-4: time per Newton
+    step (autotuned at h = 1e-3), its Jacobian against the oracle's sampled
+    columns and its step against the oracle's step; then Newton from the seeded
+    input with h = 1e-3 and the reference's damping, to mismatch 1e-12."""
+    N = shape.Ntau
+    out = {"policy": "CC_FP32_FD: every column (F32(x + h e_j) - F32(x)) / h of FP32 shoots; "
+                     "F(x) FP64; FP32 LU + FP64 refinement", "legs": {}}
+    for h in (1e-3, 1e-4):
+        r = time_policy(cc, dev, shape, cc.CC_FP32_FD, steps, 1, False, tune if h == 1e-3 else False, eps=h)
+        leg = {"ms_per_step": round(r["ms_per_step"], 2), "steps_per_s": round(r["steps_per_s"], 4),
+               "refinement": {"residual": r["refine"][0], "fp64_resolve": r["refine"][1]},
+               "schedule": r["schedule"]}
+        if ocols is not None:
+            leg["columns_vs_oracle"] = column_parity(r["J"], ocols["J"], ocols["cols"], replace(shape, EpsNewton=h))
+        if dxo is not None:
+            leg["dx_vs_oracle_step"] = step_parity(r["dx"], dxo, N)
+        out["legs"][f"h={h:g}"] = leg
+        if h == 1e-3:
+            out["mixed_over_fp32_fd"] = round(mixed_steps_per_s / r["steps_per_s"], 3)
+    out["newton_h1e-3"] = newton_study(cc, shape, ["fp32_fd"], 1e-12, 12, False)["fp32_fd"]
+    return out
+
+
 def copy_calibration(nbytes):
     """A plain device copy of nbytes (torch's copy kernel), for scaling the
     TCC FETCH_SIZE / WRITE_SIZE counters of a --pmc run to bytes."""
@@ -443,7 +475,6 @@ def main():
     if args.newton:
         shape = W.config(args.config)
         if args.scale is not None:
-            from dataclasses import replace
             shape = replace(shape, scale=args.scale)
         res = newton_study(cc, shape, args.policies.split(","), args.tol, args.max_its, not args.no_tune,
                            args.save_failures or None)
@@ -521,6 +552,7 @@ def main():
 
     # parity against the oracle at this shape: its step (fixture) and its columns (CPU leg)
     par = {}
+    ocols, dxo = None, None
     fxp = os.path.join(GOLDEN, f"step_c{args.config}_oracle.json")
     if os.path.exists(fxp):
         with open(fxp) as f:
@@ -556,6 +588,8 @@ def main():
         out["newton"] = guarded("newton to convergence", lambda: dict(
             tol=1e-12, **{"policies": newton_study(cc, shape, ["mixed", "fp64"] if fp64 is not None else ["mixed"],
                                                    1e-12, 12, False, schedules=scheds)}))
+        out["fp32_fd"] = guarded("fp32 finite differences", lambda: fp32_fd_leg(
+            cc, dev, shape, max(2, args.steps // 4), tune, ocols, dxo, mixed["steps_per_s"]))
         out["fp64_ntau512_x6000"] = guarded("fp64 at Ntau=512", lambda: fp64_leg(
             cc, dev, W.config(2), max(2, args.steps // 4), 1, tune))
         if "error" not in out["fp64_ntau512_x6000"] and not args.no_cpu:

@@ -40,8 +40,13 @@ typedef enum cc_precision {
     CC_FP64 = 0,            /* every column a forward difference of two FP64 shoots: the reference's arithmetic */
     CC_MIXED = 1,           /* FP64 base shoot, F(x) and Delta column; every other column the FP32
                                tangent-linear derivative of the shoot; FP32 LU + FP64 refinement */
-    CC_FP64_TANGENT = 2     /* as CC_MIXED with the tangent-linear columns and the LU in FP64:
+    CC_FP64_TANGENT = 2,    /* as CC_MIXED with the tangent-linear columns and the LU in FP64:
                                an accurate all-FP64 Jacobian (no counterpart in the reference) */
+    CC_FP32_FD = 3          /* every column a forward difference of two FP32 shoots (FP32 chains at
+                               PrecisionIRK32, step EpsNewton: set it far above the reference's 1e-8,
+                               which FP32 rounding swamps); F(x) an FP64 shoot; FP32 LU + FP64
+                               refinement.  The literal FP32 form of the reference's Jacobian, kept
+                               to measure against CC_MIXED (two-stage scheme only) */
 } cc_precision;
 
 /* Problem description: the SimulationConfig keys the path reads
@@ -89,7 +94,8 @@ cc_status cc_residual(cc_ctx *ctx, const double *x, double *F);
  * the reference.  CC_MIXED: F0 is not read; the Delta column is that forward
  * difference in FP64 around an FP64 base shoot, every other column is the
  * FP32 tangent-linear derivative of the shoot along e_i, integrated around
- * the recorded FP64 base trajectory. */
+ * the recorded FP64 base trajectory.  CC_FP32_FD: F0 is not read; every
+ * column is (F32(x + EpsNewton e_i) - F32(x)) / EpsNewton of FP32 shoots. */
 cc_status cc_jacobian(cc_ctx *ctx, const double *x, const double *F0, double *J, int32_t *col_status);
 
 /* Dense solve with partial pivoting: replaces LAPACKE_dgesv

@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "libcc_gpu.so")
 HEADER = os.path.join(ROOT, "include", "cc_gpu.h")
 
 CC_OK, CC_IRK_NOCONV, CC_DELTA_MISMATCH, CC_SINGULAR, CC_BADARG, CC_DEVICE = range(6)
-CC_FP64, CC_MIXED, CC_FP64_TANGENT = 0, 1, 2
+CC_FP64, CC_MIXED, CC_FP64_TANGENT, CC_FP32_FD = 0, 1, 2, 3
 STATUS_NAMES = {0: "OK", 1: "IRK_NOCONV", 2: "DELTA_MISMATCH", 3: "SINGULAR", 4: "BADARG", 5: "DEVICE"}
 
 

@@ -444,40 +444,74 @@ ccd::ChainParams<double> chain_params(const cc_ctx *c, const ColumnBuffers &B)
     return a;
 }
 
-// Clears the status words of a launch and writes the boundary states.
+// The same launch in FP32 (CC_FP32_FD): the chain states of B.Y as float, the
+// FP32 twiddles and IRK tolerance PrecisionIRK32; Delta stays FP64 per chain.
 template <int N>
-cc_status launch_prologue(cc_ctx *c, const ColumnBuffers &B, Counts k, const double *d_x, double eps, hipStream_t s)
+ccd::ChainParams<float> chain_params32(const cc_ctx *c, const ColumnBuffers &B)
+{
+    const ccd::ChainParams<double> d = chain_params<N>(c, B);
+    ccd::ChainParams<float> a{};
+    a.Y = static_cast<cx<float> *>(B.Y);
+    a.start = d.start;
+    a.nsteps = d.nsteps;
+    a.dir = d.dir;
+    a.delta = d.delta;
+    a.grid = d.grid;
+    a.status = d.status;
+    a.fail_idx = d.fail_idx;
+    a.its = d.its;
+    a.tw = c->tw32;
+    a.invm = c->invm32;
+    a.maxIts = d.maxIts;
+    a.Dim = d.Dim;
+    a.precision = c->p.PrecisionIRK32;
+    set_tableau(a, c);
+    a.rec = nullptr;
+    a.rec_base = nullptr;
+    return a;
+}
+
+// Clears the status words of a launch and writes the boundary states (as
+// float chains when fp32).
+template <int N>
+cc_status launch_prologue(cc_ctx *c, const ColumnBuffers &B, Counts k, const double *d_x, double eps, hipStream_t s,
+                          bool fp32 = false)
 {
     CCCHK(fill<int32_t>(c, B.status, 0, k.nch, s));
     CCCHK(fill<int32_t>(c, B.fail, -1, k.nch, s));
     CCCHK(fill<int64_t>(c, B.its, 0, k.nch, s));
     CCCHK(fill<int32_t>(c, B.mstatus, 0, k.ncols, s));
     if (k.nl)
-        HIPCHK(c, Ops<N>::boundary(s, true, false, k.nl, d_x, B.lcols, B.lch, eps, c->p.XLeft, c->p.Dim, c->tw64,
+        HIPCHK(c, Ops<N>::boundary(s, true, fp32, k.nl, d_x, B.lcols, B.lch, eps, c->p.XLeft, c->p.Dim, c->tw64,
                                    B.Y, B.delta));
     if (k.nr)
-        HIPCHK(c, Ops<N>::boundary(s, false, false, k.nr, d_x, B.rcols, B.rch, eps, c->p.XRight, c->p.Dim, c->tw64,
+        HIPCHK(c, Ops<N>::boundary(s, false, fp32, k.nr, d_x, B.rcols, B.rch, eps, c->p.XRight, c->p.Dim, c->tw64,
                                    B.Y, B.delta));
     return CC_OK;
 }
 
-// Boundary -> chains (in chunks of steps_per_launch) -> mismatch, on stream s.
+// Boundary -> chains (in chunks of steps_per_launch) -> mismatch, on stream s;
+// FP64 chains of T threads, or FP32 chains (of the default thread count).
+// timed: the chain launches count in cc_chain_timing.
 template <int N>
 cc_status launch_columns(cc_ctx *c, const ColumnBuffers &B, Counts k, const double *d_x, double eps, int T,
-                         hipStream_t s)
+                         hipStream_t s, bool fp32 = false, bool timed = true)
 {
-    CCCHK(launch_prologue<N>(c, B, k, d_x, eps, s));
-    ccd::ChainParams<double> a = chain_params<N>(c, B);
+    CCCHK(launch_prologue<N>(c, B, k, d_x, eps, s, fp32));
     const int maxsteps = max_chain_steps(c), S = c->sched.steps_per_launch;
-    CCCHK(time_mark(c, s, true));
-    for (int s0 = 0; s0 < maxsteps; s0 += S) {
-        a.step_begin = s0;
-        a.step_end = std::min(maxsteps, s0 + S);
-        HIPCHK(c, Ops<N>::chain(s, T, k.nch, a));
-        ++c->prof_launches;
-    }
-    CCCHK(time_mark(c, s, false));
-    HIPCHK(c, Ops<N>::mismatch(s, false, k.ncols, B.Y, B.lidx, B.ridx, B.delta, c->p.PrecisionIRK, c->tw64, B.F,
+    if (timed) CCCHK(time_mark(c, s, true));
+    auto run = [&](auto a) -> cc_status {
+        for (int s0 = 0; s0 < maxsteps; s0 += S) {
+            a.step_begin = s0;
+            a.step_end = std::min(maxsteps, s0 + S);
+            HIPCHK(c, Ops<N>::chain(s, fp32 ? Ops<N>::kThreads : T, k.nch, a));
+            if (timed) ++c->prof_launches;
+        }
+        return CC_OK;
+    };
+    CCCHK(fp32 ? run(chain_params32<N>(c, B)) : run(chain_params<N>(c, B)));
+    if (timed) CCCHK(time_mark(c, s, false));
+    HIPCHK(c, Ops<N>::mismatch(s, fp32, k.ncols, B.Y, B.lidx, B.ridx, B.delta, c->p.PrecisionIRK, c->tw64, B.F,
                                B.mstatus));
     return CC_OK;
 }
@@ -561,11 +595,14 @@ cc_status lu_factor(cc_ctx *c, R *dA, int n, hipStream_t s)
     return CC_OK;
 }
 
+// Policies that solve with the FP32 LU and FP64 refinement (else the FP64 LU).
+bool fp32_solve(const cc_ctx *c) { return c->p.precision == CC_MIXED || c->p.precision == CC_FP32_FD; }
+
 // Enqueues d_sol = J^{-1} d_rhs; LAPACK's info lands in hs.info.
 cc_status enqueue_solve(cc_ctx *c, int n, const double *d_J, const double *d_rhs, double *d_sol, hipStream_t s)
 {
     CCCHK(fill<int32_t>(c, c->info, 0, 1, s));
-    const bool mixed = c->p.precision == CC_MIXED;
+    const bool mixed = fp32_solve(c);
     const size_t nn = (size_t)n * n;
     const unsigned g = (unsigned)((nn + 255) / 256);
     if (!mixed) {
@@ -596,7 +633,7 @@ cc_status enqueue_solve(cc_ctx *c, int n, const double *d_J, const double *d_rhs
     return CC_OK;
 }
 
-// After the stream of a CC_MIXED solve synchronised: when the FP64-refined
+// After the stream of an FP32-LU solve (CC_MIXED, CC_FP32_FD) synchronised: when the FP64-refined
 // FP32 solve of J sol = rhs left a relative residual above refine_tol (or a
 // non-finite one, or the FP32 factorization met a zero pivot), the system is
 // solved again with the FP64 LU, synchronously on s, and *resolved is set.
@@ -607,7 +644,7 @@ cc_status refine_or_resolve(cc_ctx *c, int n, const double *d_J, const double *d
     *resolved = false;
     c->last_refine = 0.0;
     c->last_resolve = 0;
-    if (c->p.precision != CC_MIXED) return CC_OK;
+    if (!fp32_solve(c)) return CC_OK;
     c->last_refine = c->hs.refine[0];
     if (c->hs.refine[0] <= c->p.refine_tol && c->hs.info[0] == 0) return CC_OK;
     CCCHK(fill<int32_t>(c, c->info, 0, 1, s));
@@ -817,6 +854,42 @@ cc_status enqueue_fp64_columns(cc_ctx *c, const double *d_x)
     return st;
 }
 
+// CC_FP32_FD, the FP32 form of the reference's Jacobian: every column,
+// the base column included, an FP32 shoot at x + EpsNewton e_j, J = (F32_j -
+// F32_base) / EpsNewton (the differences of FP32 shoots, NewtonSolver.cpp:
+// 615-702); F0 = the FP64 base shoot (base plan), on s1 beside the FP32 launch
+// when the schedule pipelines, else before it on s0.
+cc_status enqueue_fp32_fd(cc_ctx *c, const double *d_x, double *d_F0)
+{
+    const int n = c->n;
+    const Counts kb = counts_of(c->base_plan), km = counts_of(c->main_plan);
+    const bool fork = c->sched.pipeline != 0;
+    hipStream_t sb = fork ? c->s1 : c->s0;
+    if (fork) {
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->s0));
+        HIPCHK(c, hipStreamWaitEvent(sb, c->ev_fork, 0));
+    }
+    cc_status st = CC_OK;
+    dispatch_n(c->N, [&](auto Nc) {
+        constexpr int N = decltype(Nc)::value;
+        st = launch_columns<N>(c, c->base, kb, d_x, 0.0, c->sched.threads_base, sb, false, false);
+        if (st == CC_OK) st = readback_columns(c, c->base, kb, true, sb);
+        if (st == CC_OK) st = launch_columns<N>(c, c->main, km, d_x, c->p.EpsNewton, Ops<N>::kThreads, c->s0, true);
+    });
+    CCCHK(st);
+    const int ncols = (int)c->main_plan.cols.size();
+    hipLaunchKernelGGL(ccd::k_fd, dim3((ncols + 127) / 128, n), dim3(128), 0, c->s0, c->main.F, c->main.cols, ncols,
+                       0, n, c->p.EpsNewton, c->J);
+    HIPCHK(c, hipGetLastError());
+    CCCHK(readback_columns(c, c->main, km, false, c->s0));
+    if (fork) {
+        HIPCHK(c, hipEventRecord(c->ev_join, sb));
+        HIPCHK(c, hipStreamWaitEvent(c->s0, c->ev_join, 0));
+    }
+    if (d_F0) HIPCHK(c, hipMemcpyAsync(d_F0, c->base.F, sizeof(double) * n, hipMemcpyDeviceToDevice, c->s0));
+    return CC_OK;
+}
+
 cc_status eval_fp64(cc_ctx *c, int32_t *col_status)
 {
     const Counts k = counts_of(c->main_plan);
@@ -831,6 +904,14 @@ cc_status eval_fp64(cc_ctx *c, int32_t *col_status)
     return st;
 }
 
+// CC_FP32_FD: the FP64 base shoot's status, then the FP32 columns'.
+cc_status eval_fp32_fd(cc_ctx *c, int32_t *col_status)
+{
+    const cc_status sb = eval_columns(c, c->base_plan, counts_of(c->base_plan), true, nullptr, true);
+    const cc_status sm = eval_fp64(c, col_status);
+    return sb != CC_OK ? sb : sm;
+}
+
 // The whole Newton step, enqueued on s0 (s1 joins in the pipelined mixed
 // form): F0 = F(x), J, d = -F0, LU, solve (+ refinement), status readback.
 cc_status enqueue_newton(cc_ctx *c, const double *d_x, double *d_F0, double *d_dx)
@@ -839,6 +920,8 @@ cc_status enqueue_newton(cc_ctx *c, const double *d_x, double *d_F0, double *d_d
     if (tangent_policy(c)) {
         CCCHK(enqueue_tangent(c, d_x));
         HIPCHK(c, hipMemcpyAsync(d_F0, c->base.F, sizeof(double) * n, hipMemcpyDeviceToDevice, c->s0));
+    } else if (c->p.precision == CC_FP32_FD) {
+        CCCHK(enqueue_fp32_fd(c, d_x, d_F0));
     } else {
         CCCHK(enqueue_fp64_columns(c, d_x));
         const int ncols = (int)c->main_plan.cols.size();
@@ -873,7 +956,7 @@ cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double 
     if (c->sched.use_graph) {
         if (!c->gexec || c->gkey[0] != d_x || c->gkey[1] != d_F0 || c->gkey[2] != d_dx) {
             drop_graph(c);
-            if (c->sched.pipeline && tangent_policy(c)) {
+            if (c->sched.pipeline && tangent_policy(c)) {      // per-chunk events of the pipeline
                 const int S = c->sched.steps_per_launch;
                 CCCHK(ensure_chunk_events(c, (max_chain_steps(c) + S - 1) / S));
             }
@@ -902,7 +985,9 @@ cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double 
     }
     if (post) CCCHK((*post)());
     HIPCHK(c, hipStreamSynchronize(c->s0));
-    const cc_status st = tangent_policy(c) ? eval_tangent(c, nullptr) : eval_fp64(c, nullptr);
+    const cc_status st = tangent_policy(c)                  ? eval_tangent(c, nullptr)
+                         : c->p.precision == CC_FP32_FD ? eval_fp32_fd(c, nullptr)
+                                                        : eval_fp64(c, nullptr);
     CCCHK(st);
     bool res = false;
     CCCHK(refine_or_resolve(c, c->n, c->J, c->d, d_dx, c->s0, &res));
@@ -914,7 +999,7 @@ cc_status newton_step_device(cc_ctx *c, const double *d_x, double *d_F0, double 
 // cleanly, so that F0 is a valid residual whatever the other columns did.
 bool base_residual_ok(const cc_ctx *c)
 {
-    if (tangent_policy(c)) {
+    if (tangent_policy(c) || c->p.precision == CC_FP32_FD) {
         const Plan &P = c->base_plan;
         return c->hs.bstatus[P.lidx[0]] == 0 && c->hs.bstatus[P.ridx[0]] == 0 && c->hs.bmstatus[0] == 0;
     }
@@ -1003,7 +1088,7 @@ cc_status tune_threads_n(cc_ctx *c, const double *d_x, std::vector<double> &time
         }
         pick(c->sched.threads_tangent, t);
         times.insert(times.end(), t.begin(), t.end());
-    } else if (!t64) {
+    } else if (!t64 && c->p.precision != CC_FP32_FD) {      // FP32 chains: default threads only
         // every column of the FP64 Jacobian: a slice of steps [4, 4 + Sk)
         const Counts km = counts_of(c->main_plan);
         const int Sk = std::min(maxsteps - 4, 48);
@@ -1048,10 +1133,12 @@ cc_status tune_stages(cc_ctx *c, std::vector<double> &times)
     // pipelined: short chunks let the last tangent chunk follow the base's end
     // closely (at Ntau=1024, 8000 x-points: 417 ms at 64 steps, 400 ms at 16);
     // one stream: long chunks, fewer launches
+    // (CC_FP32_FD pipelines the FP64 base shoot beside the whole FP32 launch: long chunks)
+    const bool fd32 = c->p.precision == CC_FP32_FD;
     for (int pipe : {1, 0}) {
-        if (pipe && !mixed) continue;
-        const std::vector<int> cand = pipe ? std::vector<int>{8, 12, 16, 24, 32, 64}
-                                           : std::vector<int>{64, 128, 256, 512, 1024};
+        if (pipe && !mixed && !fd32) continue;
+        const std::vector<int> cand = pipe && mixed ? std::vector<int>{8, 12, 16, 24, 32, 64}
+                                                    : std::vector<int>{64, 128, 256, 512, 1024};
         for (int S : cand) {
             if ((max_chain_steps(c) + S - 1) / S > kMaxChunks) continue;
             drop_graph(c);
@@ -1360,7 +1447,9 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
     if (p->SchemeIRK < 1 || p->SchemeIRK > 3 || p->NLeft < 1 || p->NRight < 1 || p->MaxIterIRK < 1) return CC_BADARG;
     // the tangent-linear columns and the batch are built for the two-stage scheme
     if (p->SchemeIRK != 2 && p->precision != CC_FP64) return CC_BADARG;
-    if (p->precision != CC_FP64 && p->precision != CC_MIXED && p->precision != CC_FP64_TANGENT) return CC_BADARG;
+    if (p->precision != CC_FP64 && p->precision != CC_MIXED && p->precision != CC_FP64_TANGENT &&
+        p->precision != CC_FP32_FD)
+        return CC_BADARG;
     if (!(p->XLeft > 0.0 && p->XLeft < p->XMid && p->XMid < p->XRight && p->XRight < 1.0)) return CC_BADARG;
     cc_ctx *c = new cc_ctx();
     c->p = *p;
@@ -1713,6 +1802,7 @@ cc_status cc_set_schedule(cc_ctx *c, const cc_schedule *s)
         !valid_threads(c->N, s->threads_tangent) ||
         (c->p.precision == CC_FP64_TANGENT && (s->threads_base != ccd::default_threads(c->N) ||
                                                s->threads_tangent != ccd::default_threads(c->N))) ||
+        (c->p.precision == CC_FP32_FD && s->threads_bulk != ccd::default_threads(c->N)) ||
         (c->stages != 2 && (s->threads_base != ccd::default_threads(c->N) ||
                             s->threads_bulk != ccd::default_threads(c->N)))) {
         c->err = "unsupported thread count for this Ntau, precision and scheme";
@@ -1789,6 +1879,13 @@ cc_status cc_jacobian(cc_ctx *c, const double *x, const double *F0, double *J, i
         HIPCHK(c, hipMemcpyAsync(J, c->J, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->s0));
         HIPCHK(c, hipStreamSynchronize(c->s0));
         st = eval_tangent(c, col_status);
+    } else if (c->p.precision == CC_FP32_FD) {
+        // differences of FP32 shoots against the FP32 base column (an FP64 F0
+        // against FP32 shoots would difference two precisions)
+        CCCHK(enqueue_fp32_fd(c, c->x, nullptr));
+        HIPCHK(c, hipMemcpyAsync(J, c->J, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->s0));
+        HIPCHK(c, hipStreamSynchronize(c->s0));
+        st = eval_fp32_fd(c, col_status);
     } else {
         // the base column is part of the launch; the difference quotient uses
         // the caller's F0 (NewtonSolver.cpp:684), stored after the last column
@@ -1816,7 +1913,7 @@ cc_status cc_solve(cc_ctx *c, int32_t n, double *A, double *b, int32_t *info)
     HIPCHK(c, hipStreamSynchronize(c->s0));
     bool resolved = false;
     CCCHK(refine_or_resolve(c, n, c->J, c->d, c->dx, c->s0, &resolved));
-    if (c->p.precision == CC_MIXED && !resolved) {
+    if (fp32_solve(c) && !resolved) {
         // FP32 factors, returned widened to double
         const unsigned g = (unsigned)(((size_t)n * n + 255) / 256);
         hipLaunchKernelGGL((ccd::k_cast_from<float>), dim3(g), dim3(256), 0, c->s0, c->A32, c->A, (size_t)n * n);
@@ -1868,7 +1965,7 @@ cc_status cc_newton_run(cc_ctx *c, double *x, double precision_newton, double sl
         const std::function<cc_status()> post = [&]() -> cc_status {
             hipLaunchKernelGGL(ccd::k_newton_update, dim3(1), dim3(256), 0, c->s0, c->x, c->xold, c->F0, c->dx, n,
                                precision_newton, slow_error, c->nst, c->nist, hist, its,
-                               c->p.precision == CC_MIXED ? (const int32_t *)c->rhold : nullptr);
+                               fp32_solve(c) ? (const int32_t *)c->rhold : nullptr);
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipMemcpyAsync(c->hs.newton, c->nist + 1, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, c->s0));
             return CC_OK;

@@ -9,6 +9,8 @@ what is computed.
   at the usual tolerances, FP32 tangent columns agree with the default
   thread count to FP32 accuracy.
 """
+from dataclasses import replace
+
 import numpy as np
 import pytest
 
@@ -31,11 +33,13 @@ def shape_c2():
     return {"shape": shape, "cfg": cfg, "x": W.initial_x(shape)}
 
 
-@pytest.mark.parametrize("precision", [cc.CC_FP64, cc.CC_MIXED])
+@pytest.mark.parametrize("precision", [cc.CC_FP64, cc.CC_MIXED, cc.CC_FP32_FD])
 def test_graph_replay_matches_eager(precision):
     """Three replays of the captured Newton step, with eager work (a residual
     and a synchronisation) between them, each bit-identical to the eager step."""
     s = shape_c1(128)
+    if precision == cc.CC_FP32_FD:          # an FD step FP32 shoots resolve (1e-8 leaves J singular)
+        s = dict(s, shape=replace(s["shape"], EpsNewton=1e-3))
     ctx = ctx_for(s, precision=precision)
     F0_e, dx_e = ctx.newton_step(s["x"])
     ctx.set_schedule(use_graph=1)
