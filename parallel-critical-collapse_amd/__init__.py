@@ -275,8 +275,9 @@ class Context:
         self._check(load().cc_newton_step_device_async(self.h, _vp(d_x), _vp(d_F0), _vp(d_dx)),
                     "cc_newton_step_device_async")
 
-    def profile(self, enable=True):
-        self._check(load().cc_profile(self.h, int(enable)), "cc_profile")
+    def profile(self, enable=True, base=True):
+        """Time the chain launches (and, with base, the tangent policies' base chains)."""
+        self._check(load().cc_profile(self.h, 0 if not enable else (1 if base else 2)), "cc_profile")
 
     def chain_timing(self):
         """(ms, launches, fixed-point iterations, IRK steps) of the last profiled launch set."""

@@ -129,7 +129,7 @@ struct cc_ctx {
     StatusBlock hs{};
     std::vector<void *> allocs;
     // kernel timing on the main stream: event pairs around chain launches
-    bool profile = false, capturing = false;
+    bool profile = false, profile_base = false, capturing = false;
     hipEvent_t tk0 = nullptr, tk1 = nullptr;
     std::vector<hipEvent_t> tev;
     int prof_pairs = 0;
@@ -376,7 +376,7 @@ cc_status time_mark(cc_ctx *c, hipStream_t s, bool begin)
 // Event pair around one base-chain launch of the tangent policies, on its stream s.
 cc_status time_mark_base(cc_ctx *c, hipStream_t s, bool begin)
 {
-    if (!c->profile || c->capturing) return CC_OK;
+    if (!c->profile || !c->profile_base || c->capturing) return CC_OK;
     const size_t idx = 2 * (size_t)c->bprof_pairs + (begin ? 0 : 1);
     while (c->bev.size() <= idx) {
         hipEvent_t e;
@@ -1740,8 +1740,9 @@ cc_status cc_refinement(cc_ctx *c, double *residual, int32_t *fp64_resolve)
 
 cc_status cc_profile(cc_ctx *c, int32_t enable)
 {
-    if (!c) return CC_BADARG;
+    if (!c || enable < 0 || enable > 2) return CC_BADARG;
     c->profile = enable != 0;
+    c->profile_base = enable == 1;
     c->prof_pending = false;
     return CC_OK;
 }
@@ -1770,7 +1771,7 @@ cc_status cc_chain_timing(cc_ctx *c, double *ms, int32_t *launches, int64_t *its
 cc_status cc_base_timing(cc_ctx *c, double *ms, int32_t *launches, int64_t *its, int64_t *steps)
 {
     if (!c || !ms || !launches || !its || !steps) return CC_BADARG;
-    if (!c->prof_pending || !tangent_policy(c)) {
+    if (!c->prof_pending || !tangent_policy(c) || !c->profile_base) {
         c->err = "no profiled eager tangent-policy step since cc_profile(ctx, 1)";
         return CC_BADARG;
     }

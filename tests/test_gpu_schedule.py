@@ -143,3 +143,27 @@ def test_autotune_keeps_results(c2_oracle_residual):
     F0, dx = ctx.newton_step(s["x"])
     assert np.max(np.abs(F0 - F_ref)) / scale < 1e-9
     assert np.all(np.isfinite(dx))
+
+
+def test_profile_modes():
+    """cc_profile 1 times the tangent and the base chains' launches, 2 only the
+    tangent's (no events on the base stream), 0 none; none changes results."""
+    s = shape_c1(128)
+    ctx = ctx_for(s, precision=cc.CC_MIXED)
+    F0, dx = ctx.newton_step(s["x"])
+    for mode in (1, 2, 0):
+        ctx.profile(mode != 0, base=mode == 1)
+        F0b, dxb = ctx.newton_step(s["x"])
+        assert np.array_equal(F0b, F0) and np.array_equal(dxb, dx)
+        if mode == 0:
+            with pytest.raises(cc.CCError):
+                ctx.chain_timing()
+            continue
+        ms, launches, its, steps = ctx.chain_timing()
+        assert ms > 0 and launches > 0 and its >= steps > 0
+        if mode == 1:
+            bms, bl, bits, bsteps = ctx.base_timing()
+            assert bms > 0 and bl > 0 and bits >= bsteps > 0
+        else:
+            with pytest.raises(cc.CCError):
+                ctx.base_timing()
