@@ -31,8 +31,14 @@
 //    (SpectralTransformer.cpp:276-291), so no e^{-F} is formed;
 //  * per stage: FieldsToStateVector as one batched forward FFT of 2 signals.
 // 10 length-N transforms per iteration instead of the reference's 18.
+//
+// The transforms are the lane-swap DIF/DIT pair of cc_lanefft.hpp: fields on
+// the tau grid are in layout L (thread t holds points t + T*p), spectra in
+// layout S, where thread t holds modes kt + T*p with kt = lf_kt(t) a digit
+// reversal of t.  Every spectral pointwise operation below indexes modes
+// through kt; the states in HBM keep the natural mode order.
 #pragma once
-#include "cc_device.hpp"
+#include "cc_lanefft.hpp"
 
 namespace ccd {
 
@@ -115,16 +121,17 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     asm volatile("" : "+v"(t));
     // DB: the transforms of one evaluation follow each other without trailing
     // barriers, each starting in the region its predecessor did not read last
-    // (fft_next_par); the spectrum separation takes the next region and the
+    // (lf_next_par); the spectrum separation takes the next region and the
     // transform after it the other one.  The caller's reduction barrier
     // separates the last transform from the next evaluation.
     constexpr bool MERGE = rhs_merge<N, R, DB>();
     constexpr int RS = rhs_region<N, R, DB>();
-    constexpr int pA = 0, pB = DB ? fft_next_par<N, T>(pA) : 0;
-    constexpr int pC = DB ? fft_next_par<N, T>(MERGE ? pA : pB) : 0;
-    constexpr int pD = DB ? fft_next_par<N, T>(pC) : 0, pE = DB ? fft_next_par<N, T>(pD) : 0;
-    constexpr int pS = DB ? fft_next_par<N, T>(pE) : 0, pG = DB ? 1 - pS : 0;
-    constexpr int pH = DB ? fft_next_par<N, T>(pG) : 0;
+    constexpr int pA = 0, pB = DB ? lf_next_par<N, T>(pA) : 0;
+    constexpr int pC = DB ? lf_next_par<N, T>(MERGE ? pA : pB) : 0;
+    constexpr int pD = DB ? lf_next_par<N, T>(pC) : 0, pE = DB ? lf_next_par<N, T>(pD) : 0;
+    constexpr int pS = DB ? lf_next_par<N, T>(pE) : 0, pG = DB ? 1 - pS : 0;
+    constexpr int pH = DB ? lf_next_par<N, T>(pG) : 0;
+    const int kt = lf_kt<N, T>(t);      // spectral modes of this thread: kt + T*p
 
     // per-stage products of the split that survive until the pointwise RHS
     R AU[2][H], BU[2][H], AV[2][H], BV[2][H], CF[2][H];
@@ -169,7 +176,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         const R fix = bc[s];
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            const int q = t + T * h;
+            const int q = kt + T * h;
             const cx<R> Y = yk[s][h];
             cx<R> lo, hi;
             if (q < M / 2) {
@@ -188,20 +195,20 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         }
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const int k = t + T * p;
+            const int k = kt + T * p;
             const R w = (k == N / 2) ? (R)0 : (R)wavenumber<N>(k) * K.k0;
             const cx<R> v = z[0][p];
             z[1][p] = {w * v.y, -(w * v.x)};
         }
         // (2) back to tau (804-805)
         if constexpr (!MERGE) {
-            if (s == 0) fft_block<N, T, -1, 2, R, DB, pA, 0, RS>(z, xb, tw, t);
-            else fft_block<N, T, -1, 2, R, DB, pB, 0, RS>(z, xb, tw, t);
+            if (s == 0) lfft_dit<N, T, -1, 2, R, DB, pA, RS>(z, xb, tw, t);
+            else lfft_dit<N, T, -1, 2, R, DB, pB, RS>(z, xb, tw, t);
             split(s, z);
         }
     }
     if constexpr (MERGE) {
-        fft_block<N, T, -1, 4, R, DB, pA, 0, RS>(*reinterpret_cast<cx<R>(*)[4][P]>(&zz), xb, tw, t);
+        lfft_dit<N, T, -1, 4, R, DB, pA, RS>(*reinterpret_cast<cx<R>(*)[4][P]>(&zz), xb, tw, t);
         split(0, zz[0]);
         split(1, zz[1]);
     }
@@ -209,7 +216,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     for (int h = 0; h < H; ++h) c[h + H] = c[h];
 
     // (4) solveInhom step 1 for both stages: F' = Coeff1 (264-270)
-    fft1<N, T, +1, R, DB, pC, 0, RS>(c, xb, tw, t);
+    lfft_dif1<N, T, +1, R, DB, pC, RS>(c, xb, tw, t);
 #pragma unroll
     for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
     if (t == 0) {                       // lambda_s = mean of Coeff1_s
@@ -218,11 +225,11 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {      // lamIntegrate, lambda = 0: G = f / (-i m k0)
-        const R im = invm[t + T * p] * K.inv_k0;
+        const R im = invm[kt + T * p] * K.inv_k0;
         const cx<R> v = c[p];
         c[p] = {-(v.y * im), v.x * im};
     }
-    fft1<N, T, -1, R, DB, pD, 0, RS>(c, xb, tw, t);
+    lfft_dit1<N, T, -1, R, DB, pD, RS>(c, xb, tw, t);
     // (5) h = -g e^{F} (276-279), forward (283); keep e^{F}
     R E[2][P];
 #pragma unroll
@@ -231,16 +238,16 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         E[1][p] = exp(c[p].y);
         c[p] = {K.mg * E[0][p], K.mg * E[1][p]};
     }
-    fft1<N, T, +1, R, DB, pE, 0, RS>(c, xb, tw, t);
+    lfft_dif1<N, T, +1, R, DB, pE, RS>(c, xb, tw, t);
 #pragma unroll
     for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
     // (6) separate the two Hermitian spectra, lamIntegrate with lambda_s (284)
 #pragma unroll
-    for (int p = 0; p < P; ++p) xb[pS * RS + t + T * p] = c[p];
+    for (int p = 0; p < P; ++p) xb[pS * RS + lf_mirror_pos<N, T, R>(kt + T * p)] = c[p];
     __syncthreads();
     cx<R> cm[P];
 #pragma unroll
-    for (int p = 0; p < P; ++p) cm[p] = xb[pS * RS + ((N - (t + T * p)) & (N - 1))];
+    for (int p = 0; p < P; ++p) cm[p] = xb[pS * RS + lf_mirror_pos<N, T, R>((N - (kt + T * p)) & (N - 1))];
     if constexpr (!DB) __syncthreads();
     const R lam0 = bc[2], lam1 = bc[3];
     if (rec && t == 0) {
@@ -249,7 +256,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-        const int k = t + T * p;
+        const int k = kt + T * p;
         const cx<R> A = c[p];
         const cx<R> B = {cm[p].x, -cm[p].y};
         const cx<R> S0 = cscale(A + B, half);
@@ -277,7 +284,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         }
         c[p] = {G[0].x - G[1].y, G[0].y + G[1].x};
     }
-    fft1<N, T, -1, R, DB, pG, 0, RS>(c, xb, tw, t);
+    lfft_dit1<N, T, -1, R, DB, pG, RS>(c, xb, tw, t);
 
     // (7) pointwise RHS with r = 1/IA2 = e^{F}/X (288-291; ODEStepper.cpp:109-122)
     cx<R> yp[2][P];
@@ -298,12 +305,12 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         }
     }
     // (8) FieldsToStateVector (766-778) and Re dY[2] = 0 (ODEStepper.cpp:128)
-    fft_block<N, T, +1, 2, R, DB, pH, 0, RS>(yp, xb, tw, t);
+    lfft_dif<N, T, +1, 2, R, DB, pH, RS>(yp, xb, tw, t);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            const int q = t + T * h;
+            const int q = kt + T * h;
             const cx<R> lo = cscale(yp[s][h], invN), hi = cscale(yp[s][h + H], invN);
             cx<R> v = (q < M / 2) ? lo : ((q == M / 2) ? lo + hi : hi);
             if (q == 2) v.x = (R)0;
@@ -364,11 +371,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
     cx<R> *Yg = a.Y + (size_t)ch * M;
     // The step-start state y and the candidate next state y + dx sum b_i f_i
     // are read once per iteration: they live in LDS (thread-private slots),
-    // the stage iterates in registers.
+    // the stage iterates in registers.  Thread t holds modes lf_kt(t) + T*h.
     __shared__ cx<R> ys[2][M];
     cx<R> yk[S][H];
+    constexpr int TFIX = lf_owner<N, T>(M - 2) / 64;   // holds mode M-2 in slot H-1
+    static_assert(lf_owner<N, T>(M - 2) % 64 == H - 1, "mode M-2 in the last state slot");
 #pragma unroll
-    for (int h = 0; h < H; ++h) ys[0][t + T * h] = Yg[t + T * h];
+    for (int h = 0; h < H; ++h) ys[0][t + T * h] = Yg[lf_kt<N, T>(t) + T * h];
     const RhsConst<R> K = make_const<N, R>(a.dim ? a.dim[ch] : a.Dim, a.delta[ch]);
     const double *grid = a.grid + (a.goff ? a.goff[ch] : 0);
 
@@ -391,7 +400,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
         for (int h = 0; h < H; ++h)
 #pragma unroll
             for (int i = 0; i < S; ++i) yk[i][h] = ys[0][t + T * h];
-        if (t == T - 2) {                  // Re of mode M-2 of each stage, for the fix
+        if (t == TFIX) {                   // Re of mode M-2 of each stage, for the fix
             bc[0] = bc[1] = yk[0][H - 1].x;
             if constexpr (S == 3) bc[4] = bc[5] = yk[0][H - 1].x;
         }
@@ -458,7 +467,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
                 loc += sq;
                 ys[1][t + T * h] = nn;
             }
-            if (t == T - 2) {
+            if (t == TFIX) {
                 bc[0] = yk[0][H - 1].x;
                 bc[1] = yk[S == 1 ? 0 : 1][H - 1].x;
                 if constexpr (S == 3) bc[4] = bc[5] = yk[2][H - 1].x;
@@ -483,7 +492,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
         for (int h = 0; h < H; ++h) ys[0][t + T * h] = ys[1][t + T * h];
     }
 #pragma unroll
-    for (int h = 0; h < H; ++h) Yg[t + T * h] = ys[0][t + T * h];
+    for (int h = 0; h < H; ++h) Yg[lf_kt<N, T>(t) + T * h] = ys[0][t + T * h];
     if (t == 0) {
         a.its[ch] += its_sum;
         if (st) a.status[ch] = st;
@@ -504,11 +513,12 @@ __global__ __launch_bounds__(T) void k_rhs_pairs(const cx<R> *Y, const double *x
     const int t = threadIdx.x;
     const int pr = blockIdx.x;
     cx<R> yk[2][H], f[2][H];
+    const int kt = lf_kt<N, T>(t);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int h = 0; h < H; ++h) yk[s][h] = Y[(size_t)(2 * pr + s) * M + t + T * h];
-    if (t == T - 2) {
+        for (int h = 0; h < H; ++h) yk[s][h] = Y[(size_t)(2 * pr + s) * M + kt + T * h];
+    if (t == lf_owner<N, T>(M - 2) / 64) {
         bc[0] = yk[0][H - 1].x;
         bc[1] = yk[1][H - 1].x;
     }
@@ -519,7 +529,7 @@ __global__ __launch_bounds__(T) void k_rhs_pairs(const cx<R> *Y, const double *x
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int h = 0; h < H; ++h) dY[(size_t)(2 * pr + s) * M + t + T * h] = f[s][h];
+        for (int h = 0; h < H; ++h) dY[(size_t)(2 * pr + s) * M + kt + T * h] = f[s][h];
 }
 
 }  // namespace ccd
