@@ -14,7 +14,8 @@
 // finite differences drown in rounding (the Jacobian at these shapes has
 // condition numbers near 1e9).
 //
-// Linearised pieces (all in layout L, both stages evaluated together):
+// Linearised pieces (fields in layout L, spectra in layout S as in
+// cc_chain.hpp; both stages evaluated together):
 //   fields:  the same doubled spectrum / mode-2 fix / tau-derivative / inverse
 //            transform as the base, applied to dY (linear);
 //   IA2:     IA2 solves IA2' + C1 IA2 + (D-3) = 0 periodically, so
@@ -81,9 +82,10 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
     // exchange regions of the chained transforms (as rhs_pair): no trailing
     // barriers; the caller's reduction barrier ends the evaluation
     constexpr int DB = tangent_db<N, R>(), RS = 2 * N;
-    constexpr int pA = 0, pB = DB ? fft_next_par<N, T>(pA) : 0, pC = DB ? fft_next_par<N, T>(pB) : 0;
-    constexpr int pS = DB ? fft_next_par<N, T>(pC) : 0, pG = DB ? 1 - pS : 0;
-    constexpr int pH = DB ? fft_next_par<N, T>(pG) : 0;
+    constexpr int pA = 0, pB = DB ? lf_next_par<N, T>(pA) : 0, pC = DB ? lf_next_par<N, T>(pB) : 0;
+    constexpr int pS = DB ? lf_next_par<N, T>(pC) : 0, pG = DB ? 1 - pS : 0;
+    constexpr int pH = DB ? lf_next_par<N, T>(pG) : 0;
+    const int kt = lf_kt<N, T>(t);      // spectral modes of this thread: kt + T*p
     // per-stage products that survive to the pointwise tangent RHS
     // (the base-only products B_U, B_V, C_F are recomputed from the record in
     // the pointwise stage: held across the transforms they would spill)
@@ -96,7 +98,7 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         const R fix = bc[s];
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            const int q = t + T * h;
+            const int q = kt + T * h;
             const cx<R> Y = dk[s][h];
             cx<R> lo, hi;
             if (q < M / 2) {
@@ -115,13 +117,13 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         }
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const int k = t + T * p;
+            const int k = kt + T * p;
             const R w = (k == N / 2) ? R(0.0) : (R)wavenumber<N>(k) * K.k0;
             const cx<R> v = z[0][p];
             z[1][p] = {w * v.y, -(w * v.x)};
         }
-        if (s == 0) fft_block<N, T, -1, 2, R, DB, pA, 0, RS>(z, xb, tw, t);
-        else fft_block<N, T, -1, 2, R, DB, pB, 0, RS>(z, xb, tw, t);
+        if (s == 0) lfft_dit<N, T, -1, 2, R, DB, pA, RS>(z, xb, tw, t);
+        else lfft_dit<N, T, -1, 2, R, DB, pB, RS>(z, xb, tw, t);
         // The record of a step is the same for every fixed-point iteration and
         // every transform; an opaque offset keeps this stage's loads behind its
         // transform instead of hoisted in front of it (or of the loop), where
@@ -163,20 +165,20 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         }
     }
     // dIA2 = solveInhom with the base's integrating factor and lambda
-    fft1<N, T, +1, R, DB, pC, 0, RS>(c, xb, tw, t);
+    lfft_dif1<N, T, +1, R, DB, pC, RS>(c, xb, tw, t);
 #pragma unroll
     for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
 #pragma unroll
-    for (int p = 0; p < P; ++p) xb[pS * RS + t + T * p] = c[p];
+    for (int p = 0; p < P; ++p) xb[pS * RS + lf_mirror_pos<N, T, R>(kt + T * p)] = c[p];
     __syncthreads();
     cx<R> cm[P];
 #pragma unroll
-    for (int p = 0; p < P; ++p) cm[p] = xb[pS * RS + ((N - (t + T * p)) & (N - 1))];
+    for (int p = 0; p < P; ++p) cm[p] = xb[pS * RS + lf_mirror_pos<N, T, R>((N - (kt + T * p)) & (N - 1))];
     if constexpr (!DB) __syncthreads();
     const R lam0 = rec[RC::LAM], lam1 = rec[RC::STAGE + RC::LAM];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-        const int k = t + T * p;
+        const int k = kt + T * p;
         const cx<R> A = c[p];
         const cx<R> B = {cm[p].x, -cm[p].y};
         const cx<R> S0 = cscale(A + B, half);
@@ -200,7 +202,7 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         }
         c[p] = {G[0].x - G[1].y, G[0].y + G[1].x};
     }
-    fft1<N, T, -1, R, DB, pG, 0, RS>(c, xb, tw, t);
+    lfft_dit1<N, T, -1, R, DB, pG, RS>(c, xb, tw, t);
     // pointwise tangent RHS: the half-period products of point jh serve both
     // points jh and jh + N/2 (U, V odd and F even under the half shift)
     cx<R> yp[2][P];
@@ -232,12 +234,12 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             }
         }
     }
-    fft_block<N, T, +1, 2, R, DB, pH, 0, RS>(yp, xb, tw, t);
+    lfft_dif<N, T, +1, 2, R, DB, pH, RS>(yp, xb, tw, t);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            const int q = t + T * h;
+            const int q = kt + T * h;
             const cx<R> lo = cscale(yp[s][h], invN), hi = cscale(yp[s][h + H], invN);
             cx<R> v = (q < M / 2) ? lo : ((q == M / 2) ? lo + hi : hi);
             if (q == 2) v.x = R(0.0);
@@ -276,8 +278,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
     const int s_end = min(a.step_end, nst);
     if (s_begin >= s_end) return;
     cx<R> *Yg = a.dY + (size_t)ch * M;
+    // thread t holds modes lf_kt(t) + T*h (layout S); HBM keeps the natural order
+    constexpr int TFIX = lf_owner<N, T>(M - 2) / 64;   // holds mode M-2 in slot H-1
+    static_assert(lf_owner<N, T>(M - 2) % 64 == H - 1, "mode M-2 in the last state slot");
 #pragma unroll
-    for (int h = 0; h < H; ++h) ys[0][t + T * h] = Yg[t + T * h];
+    for (int h = 0; h < H; ++h) ys[0][t + T * h] = Yg[lf_kt<N, T>(t) + T * h];
     const RhsConst<R> K = make_const<N, R>(a.dim ? a.dim[ch] : a.Dim, a.delta[bsi]);
     const double *grid = a.grid + (a.goff ? a.goff[ch] : 0);
     int idx = a.start[ch] + dir * s_begin;
@@ -290,10 +295,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
     bool have_prev = PRED && a.fprev && s_begin > 0;
     if (have_prev) {
         const cx<R> *fp = a.fprev + (size_t)ch * 2 * M;
+        const int kt = lf_kt<N, T>(t);
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            fs[0][t + T * h] = fp[t + T * h];
-            fs[1][t + T * h] = fp[M + t + T * h];
+            fs[0][t + T * h] = fp[kt + T * h];
+            fs[1][t + T * h] = fp[M + kt + T * h];
         }
     }
     for (int step = s_begin; step < s_end; ++step, idx += dir) {
@@ -333,7 +339,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
                 dk[1][h] = {y.x + E10 * f0.x + E11 * f1.x, y.y + E10 * f0.y + E11 * f1.y};
             }
         }
-        if (t == T - 2) {
+        if (t == TFIX) {
             bc[0] = dk[0][H - 1].x;
             bc[1] = dk[1][H - 1].x;
         }
@@ -362,7 +368,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
                 dk[1][h] = n1;
                 ys[1][t + T * h] = nn;
             }
-            if (t == T - 2) {
+            if (t == TFIX) {
                 bc[0] = dk[0][H - 1].x;
                 bc[1] = dk[1][H - 1].x;
             }
@@ -401,14 +407,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
     // across the step loop from the prologue
     int te = t;
     asm volatile("" : "+v"(te));
+    const int kte = lf_kt<N, T>(te);
 #pragma unroll
-    for (int h = 0; h < H; ++h) Yg[te + T * h] = ys[0][te + T * h];
+    for (int h = 0; h < H; ++h) Yg[kte + T * h] = ys[0][te + T * h];
     if (PRED && a.fprev && have_prev) {
         cx<R> *fp = a.fprev + (size_t)ch * 2 * M;
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            fp[te + T * h] = fs[0][te + T * h];
-            fp[M + te + T * h] = fs[1][te + T * h];
+            fp[kte + T * h] = fs[0][te + T * h];
+            fp[M + kte + T * h] = fs[1][te + T * h];
         }
     }
     if (t == 0) {
