@@ -103,7 +103,8 @@ constexpr int rhs_region() { return (rhs_merge<N, R, DB>() ? 4 : 2) * N; }
 template <int N, typename R, int DB>
 constexpr int rhs_lds() { return (1 + DB) * rhs_region<N, R, DB>(); }
 
-// Evaluate f_s = RHS(yk_s, xs_s) for both stages (ODEStepper.cpp:92-136).
+// Evaluate f_s = RHS(yk_s, xs_s) for both stages (ODEStepper.cpp:92-136);
+// f returns N f_s (the last forward transform's 1/N is left to the caller).
 // bc[0], bc[1]: Re of mode M-2 of each stage state (the mode-2 fix value),
 // published by the caller before the call.  xb: rhs_lds<N, R, DB>() complex.
 template <int N, int T, typename R, typename RT = float, int DB = 0>
@@ -217,15 +218,15 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
 
     // (4) solveInhom step 1 for both stages: F' = Coeff1 (264-270)
     lfft_dif1<N, T, +1, R, DB, pC, RS>(c, xb, tw, t);
-#pragma unroll
-    for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
+    // The forward transform's 1/N (a power of two: exact) is folded into the
+    // products that follow, here and below.
     if (t == 0) {                       // lambda_s = mean of Coeff1_s
-        bc[2] = c[0].x;
-        bc[3] = c[0].y;
+        bc[2] = c[0].x * invN;
+        bc[3] = c[0].y * invN;
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {      // lamIntegrate, lambda = 0: G = f / (-i m k0)
-        const R im = invm[kt + T * p] * K.inv_k0;
+        const R im = invm[kt + T * p] * (K.inv_k0 * invN);
         const cx<R> v = c[p];
         c[p] = {-(v.y * im), v.x * im};
     }
@@ -239,8 +240,6 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         c[p] = {K.mg * E[0][p], K.mg * E[1][p]};
     }
     lfft_dif1<N, T, +1, R, DB, pE, RS>(c, xb, tw, t);
-#pragma unroll
-    for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
     // (6) separate the two Hermitian spectra, lamIntegrate with lambda_s (284)
 #pragma unroll
     for (int p = 0; p < P; ++p) xb[pS * RS + lf_mirror_pos<N, T, R>(kt + T * p)] = c[p];
@@ -259,9 +258,10 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         const int k = kt + T * p;
         const cx<R> A = c[p];
         const cx<R> B = {cm[p].x, -cm[p].y};
-        const cx<R> S0 = cscale(A + B, half);
+        const R hn = half * invN;
+        const cx<R> S0 = cscale(A + B, hn);
         const cx<R> Dd = A - B;
-        const cx<R> S1 = {half * Dd.y, -(half * Dd.x)};
+        const cx<R> S1 = {hn * Dd.y, -(hn * Dd.x)};
         cx<R> G[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -274,7 +274,10 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
             } else {
                 const R w = (R)wavenumber<N>(k) * K.k0;
                 const R den = lam * lam + w * w;
-                if (sqrt(den) < (R)1e-14) {
+                // the reference's |lambda + i w| < 1e-14 (SpectralTransformer.cpp:181) as
+                // den < 1e-28: the same test without a square root (w != 0 here, so
+                // neither form is ever true at a finite period)
+                if (den < (R)1e-28) {
                     G[s] = {(R)0, (R)0};
                 } else {
                     const R inv = (R)1 / den;
@@ -311,10 +314,10 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
 #pragma unroll
         for (int h = 0; h < H; ++h) {
             const int q = kt + T * h;
-            const cx<R> lo = cscale(yp[s][h], invN), hi = cscale(yp[s][h + H], invN);
+            const cx<R> lo = yp[s][h], hi = yp[s][h + H];
             cx<R> v = (q < M / 2) ? lo : ((q == M / 2) ? lo + hi : hi);
             if (q == 2) v.x = (R)0;
-            f[s][h] = v;
+            f[s][h] = v;                    // N times the derivative: callers fold in 1/N
         }
 }
 
@@ -389,12 +392,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
         const double xin = grid[idx], xout = grid[idx + dir];
         const double dxd = xout - xin;
         R xs[S], A[S][S], B[S];
+        const R invN = (R)1 / (R)N;     // rhs_pair returns N f: exact power-of-two rescaling
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             xs[i] = (R)(xin + dxd * a.c[i]);
-            B[i] = (R)(dxd * a.b[i]);
+            B[i] = (R)(dxd * a.b[i]) * invN;
 #pragma unroll
-            for (int k = 0; k < S; ++k) A[i][k] = (R)(dxd * a.a[i][k]);
+            for (int k = 0; k < S; ++k) A[i][k] = (R)(dxd * a.a[i][k]) * invN;
         }
 #pragma unroll
         for (int h = 0; h < H; ++h)
@@ -529,7 +533,7 @@ __global__ __launch_bounds__(T) void k_rhs_pairs(const cx<R> *Y, const double *x
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int h = 0; h < H; ++h) dY[(size_t)(2 * pr + s) * M + kt + T * h] = f[s][h];
+        for (int h = 0; h < H; ++h) dY[(size_t)(2 * pr + s) * M + kt + T * h] = cscale(f[s][h], (R)1 / (R)N);
 }
 
 }  // namespace ccd

@@ -286,7 +286,18 @@ __device__ __forceinline__ void lf_lds_swap(cx<R> (&v)[B][N / T], cx<R> *lds, in
     if constexpr (!DB) __syncthreads();
 }
 
+// a * w for SIGN > 0, a * conj(w) for SIGN < 0 (tables hold e^{+2 pi i e / N})
+template <int SIGN, typename R>
+__device__ __forceinline__ cx<R> cmul_tw(cx<R> a, cx<R> w)
+{
+    if constexpr (SIGN > 0) return cmul(a, w);
+    else return {a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y};
+}
+
 // Radix 2^S butterflies over register bits [0, S) on position bits [LO, LO + S).
+// Radix <= 4: one table load and products of powers (w^2 = w w, w^3 = w w^2);
+// loading every power instead makes the chains slower (the loads' latency is
+// on the dependent path); larger radices load every power.
 template <int N, int T, int SIGN, bool DIT, int LO, int S, int B, typename R>
 __device__ __forceinline__ void lf_stage(cx<R> (&v)[B][N / T], const cx<R> *__restrict__ tw, int t)
 {
@@ -297,20 +308,14 @@ __device__ __forceinline__ void lf_stage(cx<R> (&v)[B][N / T], const cx<R> *__re
         asm volatile("" : "+v"(e));
         constexpr int STRIDE = N >> (LO + S);
         if constexpr (RAD <= 4) {
-            cx<R> w1 = tw[e * STRIDE];
-            if constexpr (SIGN < 0) w1.y = -w1.y;
-            w[1] = w1;
+            w[1] = tw[e * STRIDE];
             if constexpr (RAD == 4) {
-                w[2] = cmul(w1, w1);
-                w[3] = cmul(w1, w[2]);
+                w[2] = cmul(w[1], w[1]);
+                w[3] = cmul(w[1], w[2]);
             }
         } else {
 #pragma unroll
-            for (int d = 1; d < RAD; ++d) {
-                cx<R> tr = tw[d * e * STRIDE];
-                if constexpr (SIGN < 0) tr.y = -tr.y;
-                w[d] = tr;
-            }
+            for (int d = 1; d < RAD; ++d) w[d] = tw[d * e * STRIDE];
         }
     }
 #pragma unroll
@@ -322,12 +327,12 @@ __device__ __forceinline__ void lf_stage(cx<R> (&v)[B][N / T], const cx<R> *__re
             for (int d = 0; d < RAD; ++d) u[d] = v[s][g * RAD + d];
             if constexpr (DIT && LO > 0) {
 #pragma unroll
-                for (int d = 1; d < RAD; ++d) u[d] = cmul(u[d], w[d]);
+                for (int d = 1; d < RAD; ++d) u[d] = cmul_tw<SIGN>(u[d], w[d]);
             }
             dft<RAD, SIGN>(u);
             if constexpr (!DIT && LO > 0) {
 #pragma unroll
-                for (int d = 1; d < RAD; ++d) u[d] = cmul(u[d], w[d]);
+                for (int d = 1; d < RAD; ++d) u[d] = cmul_tw<SIGN>(u[d], w[d]);
             }
 #pragma unroll
             for (int d = 0; d < RAD; ++d) v[s][g * RAD + d] = u[d];

@@ -166,8 +166,7 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
     }
     // dIA2 = solveInhom with the base's integrating factor and lambda
     lfft_dif1<N, T, +1, R, DB, pC, RS>(c, xb, tw, t);
-#pragma unroll
-    for (int p = 0; p < P; ++p) c[p] = cscale(c[p], invN);
+    // (the transform's 1/N, a power of two, is folded into the separation below)
 #pragma unroll
     for (int p = 0; p < P; ++p) xb[pS * RS + lf_mirror_pos<N, T, R>(kt + T * p)] = c[p];
     __syncthreads();
@@ -181,9 +180,10 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         const int k = kt + T * p;
         const cx<R> A = c[p];
         const cx<R> B = {cm[p].x, -cm[p].y};
-        const cx<R> S0 = cscale(A + B, half);
+        const R hn = half * invN;
+        const cx<R> S0 = cscale(A + B, hn);
         const cx<R> Dd = A - B;
-        const cx<R> S1 = {half * Dd.y, -(half * Dd.x)};
+        const cx<R> S1 = {hn * Dd.y, -(hn * Dd.x)};
         cx<R> G[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -240,10 +240,10 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
 #pragma unroll
         for (int h = 0; h < H; ++h) {
             const int q = kt + T * h;
-            const cx<R> lo = cscale(yp[s][h], invN), hi = cscale(yp[s][h + H], invN);
+            const cx<R> lo = yp[s][h], hi = yp[s][h + H];
             cx<R> v = (q < M / 2) ? lo : ((q == M / 2) ? lo + hi : hi);
             if (q == 2) v.x = R(0.0);
-            df[s][h] = v;
+            df[s][h] = v;                   // N times the derivative: the caller folds in 1/N
         }
 }
 
@@ -306,9 +306,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
         const double xin = grid[idx], xout = grid[idx + dir];
         const double dxd = xout - xin;
         const R xs[2] = {(R)(xin + dxd * a.c0), (R)(xin + dxd * a.c1)};
-        const R A00 = (R)(dxd * a.a00), A01 = (R)(dxd * a.a01), A10 = (R)(dxd * a.a10),
-                    A11 = (R)(dxd * a.a11);
-        const R B0 = (R)(dxd * a.b0), B1 = (R)(dxd * a.b1);
+        // tangent_rhs_pair returns N times the derivatives: 1/N (exact) goes into the tableau
+        const R invN = R(1.0) / (R)N;
+        const R A00 = (R)(dxd * a.a00) * invN, A01 = (R)(dxd * a.a01) * invN, A10 = (R)(dxd * a.a10) * invN,
+                    A11 = (R)(dxd * a.a11) * invN;
+        const R B0 = (R)(dxd * a.b0) * invN, B1 = (R)(dxd * a.b1) * invN;
         const R *rec = a.rec + (size_t)(a.rec_base[ch] + step) * Rec<N>::STEP;
         // Initial stage iterates: the step-start state, or (after a chain's
         // first step) the previous step's collocation polynomial extrapolated
@@ -322,7 +324,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
         R E00 = R(0.0), E01 = R(0.0), E10 = R(0.0), E11 = R(0.0);
         if (PRED && have_prev) {
             const R r = (R)dxd * trcp((R)(xin - grid[idx - dir]));
-            const R c0 = (R)a.c0, c1 = (R)a.c1, g = R(0.5) * (R)dxd * trcp(c0 - c1);
+            // (times 1/N: the stored derivatives are N times the derivatives)
+            const R c0 = (R)a.c0, c1 = (R)a.c1, g = R(0.5) * (R)dxd * trcp(c0 - c1) * invN;
             const R u0 = R(2.0) * (R(1.0) - c0), u1 = R(2.0) * (R(1.0) - c1);
             E00 = g * c0 * (u1 + r * c0);
             E01 = -(g * c0 * (u0 + r * c0));
