@@ -133,6 +133,13 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     constexpr int pS = DB ? lf_next_par<N, T>(pE) : 0, pG = DB ? 1 - pS : 0;
     constexpr int pH = DB ? lf_next_par<N, T>(pG) : 0;
     const int kt = lf_kt<N, T>(t);      // spectral modes of this thread: kt + T*p
+    // FP64 chains of the few-chain (double-buffered) variant: the stage
+    // twiddles are loaded once per evaluation and held in registers.  The
+    // chip-filling variants have no room in their register budget (they
+    // would spill) and load per stage.
+    constexpr bool PRE = sizeof(R) == 8 && DB != 0;
+    LfTw<N, T, R> twr{};
+    if constexpr (PRE) twr = lf_twiddles<N, T, R>(tw, t);
 
     // per-stage products of the split that survive until the pointwise RHS
     R AU[2][H], BU[2][H], AV[2][H], BV[2][H], CF[2][H];
@@ -203,13 +210,13 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         }
         // (2) back to tau (804-805)
         if constexpr (!MERGE) {
-            if (s == 0) lfft_dit<N, T, -1, 2, R, DB, pA, RS>(z, xb, tw, t);
-            else lfft_dit<N, T, -1, 2, R, DB, pB, RS>(z, xb, tw, t);
+            if (s == 0) lfft_dit<N, T, -1, 2, R, DB, pA, RS, PRE>(z, xb, tw, t, &twr);
+            else lfft_dit<N, T, -1, 2, R, DB, pB, RS, PRE>(z, xb, tw, t, &twr);
             split(s, z);
         }
     }
     if constexpr (MERGE) {
-        lfft_dit<N, T, -1, 4, R, DB, pA, RS>(*reinterpret_cast<cx<R>(*)[4][P]>(&zz), xb, tw, t);
+        lfft_dit<N, T, -1, 4, R, DB, pA, RS, PRE>(*reinterpret_cast<cx<R>(*)[4][P]>(&zz), xb, tw, t, &twr);
         split(0, zz[0]);
         split(1, zz[1]);
     }
@@ -217,7 +224,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     for (int h = 0; h < H; ++h) c[h + H] = c[h];
 
     // (4) solveInhom step 1 for both stages: F' = Coeff1 (264-270)
-    lfft_dif1<N, T, +1, R, DB, pC, RS>(c, xb, tw, t);
+    lfft_dif1<N, T, +1, R, DB, pC, RS, PRE>(c, xb, tw, t, &twr);
     // The forward transform's 1/N (a power of two: exact) is folded into the
     // products that follow, here and below.
     if (t == 0) {                       // lambda_s = mean of Coeff1_s
@@ -230,7 +237,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         const cx<R> v = c[p];
         c[p] = {-(v.y * im), v.x * im};
     }
-    lfft_dit1<N, T, -1, R, DB, pD, RS>(c, xb, tw, t);
+    lfft_dit1<N, T, -1, R, DB, pD, RS, PRE>(c, xb, tw, t, &twr);
     // (5) h = -g e^{F} (276-279), forward (283); keep e^{F}
     R E[2][P];
 #pragma unroll
@@ -239,7 +246,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         E[1][p] = exp(c[p].y);
         c[p] = {K.mg * E[0][p], K.mg * E[1][p]};
     }
-    lfft_dif1<N, T, +1, R, DB, pE, RS>(c, xb, tw, t);
+    lfft_dif1<N, T, +1, R, DB, pE, RS, PRE>(c, xb, tw, t, &twr);
     // (6) separate the two Hermitian spectra, lamIntegrate with lambda_s (284)
 #pragma unroll
     for (int p = 0; p < P; ++p) xb[pS * RS + lf_mirror_pos<N, T, R>(kt + T * p)] = c[p];
@@ -287,7 +294,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         }
         c[p] = {G[0].x - G[1].y, G[0].y + G[1].x};
     }
-    lfft_dit1<N, T, -1, R, DB, pG, RS>(c, xb, tw, t);
+    lfft_dit1<N, T, -1, R, DB, pG, RS, PRE>(c, xb, tw, t, &twr);
 
     // (7) pointwise RHS with r = 1/IA2 = e^{F}/X (288-291; ODEStepper.cpp:109-122)
     cx<R> yp[2][P];
@@ -308,7 +315,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         }
     }
     // (8) FieldsToStateVector (766-778) and Re dY[2] = 0 (ODEStepper.cpp:128)
-    lfft_dif<N, T, +1, 2, R, DB, pH, RS>(yp, xb, tw, t);
+    lfft_dif<N, T, +1, 2, R, DB, pH, RS, PRE>(yp, xb, tw, t, &twr);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll

@@ -294,13 +294,37 @@ __device__ __forceinline__ cx<R> cmul_tw(cx<R> a, cx<R> w)
     else return {a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y};
 }
 
-// Radix 2^S butterflies over register bits [0, S) on position bits [LO, LO + S).
-// Radix <= 4: one table load and products of powers (w^2 = w w, w^3 = w w^2);
-// loading every power instead makes the chains slower (the loads' latency is
-// on the dependent path); larger radices load every power.
-template <int N, int T, int SIGN, bool DIT, int LO, int S, int B, typename R>
-__device__ __forceinline__ void lf_stage(cx<R> (&v)[B][N / T], const cx<R> *__restrict__ tw, int t)
+// First twiddle w = e^{2 pi i e / 2^{lo+s}} of every stage of radix <= 4 with
+// lo > 0.  DIF and DIT stages on the same digit share it (DIT, or SIGN < 0,
+// multiplies by the conjugate), so one set per thread serves every transform
+// of a workgroup; a caller that can hold it in registers loads it once
+// (lf_twiddles) and passes it to the transforms.
+template <int N, int T, typename R>
+struct LfTw {
+    cx<R> w[12];
+};
+template <int N, int T, typename R>
+__device__ __forceinline__ LfTw<N, T, R> lf_twiddles(const cx<R> *__restrict__ tw, int t)
 {
+    LfTw<N, T, R> r{};
+    sfor<0, LF<N, T>::p.nd>([&](auto Dd) {
+        constexpr int d = decltype(Dd)::value;
+        constexpr int LO = LF<N, T>::p.lo[d], S = LF<N, T>::p.sz[d];
+        if constexpr (LO > 0 && S <= 2) r.w[d] = tw[(t & ((1 << LO) - 1)) * (N >> (LO + S))];
+    });
+    return r;
+}
+
+// Radix 2^S butterflies over register bits [0, S) on position bits [LO, LO + S)
+// (DIF stage D).  Radix <= 4: one table load (or the preloaded factor, PRE)
+// and products of powers (w^2 = w w, w^3 = w w^2); loading every power instead
+// makes the chains slower (the loads' latency is on the dependent path);
+// larger radices load every power.
+template <int N, int T, int SIGN, bool DIT, int D, bool PRE, int B, typename R>
+__device__ __forceinline__ void lf_stage(cx<R> (&v)[B][N / T], const cx<R> *__restrict__ tw, int t,
+                                         const LfTw<N, T, R> *pre)
+{
+    constexpr int LO = LF<N, T>::p.lo[D], S = LF<N, T>::p.sz[D];
     constexpr int P = N / T, RAD = 1 << S, G = P / RAD;
     cx<R> w[RAD];
     if constexpr (LO > 0) {
@@ -308,7 +332,8 @@ __device__ __forceinline__ void lf_stage(cx<R> (&v)[B][N / T], const cx<R> *__re
         asm volatile("" : "+v"(e));
         constexpr int STRIDE = N >> (LO + S);
         if constexpr (RAD <= 4) {
-            w[1] = tw[e * STRIDE];
+            if constexpr (PRE) w[1] = pre->w[D];
+            else w[1] = tw[e * STRIDE];
             if constexpr (RAD == 4) {
                 w[2] = cmul(w[1], w[1]);
                 w[3] = cmul(w[1], w[2]);
@@ -352,8 +377,9 @@ constexpr int lf_lds_before(int D)
 // DB / P0: exchange regions alternate from region P0 (as fft_block); with no
 // trailing barrier the caller's next LDS writer uses region
 // lf_next_par<N, T>(P0).
-template <int N, int T, int SIGN, int B, typename R, int DB = 0, int P0 = 0, int RS = B * N>
-__device__ __forceinline__ void lfft_dif(cx<R> (&v)[B][N / T], cx<R> *lds, const cx<R> *__restrict__ tw, int t)
+template <int N, int T, int SIGN, int B, typename R, int DB = 0, int P0 = 0, int RS = B * N, bool PRE = false>
+__device__ __forceinline__ void lfft_dif(cx<R> (&v)[B][N / T], cx<R> *lds, const cx<R> *__restrict__ tw, int t,
+                                         const LfTw<N, T, R> *pre = nullptr)
 {
     constexpr LfPlan p = LF<N, T>::p;
     static_assert(DB || P0 == 0, "one exchange region without double buffering");
@@ -370,19 +396,20 @@ __device__ __forceinline__ void lfft_dif(cx<R> (&v)[B][N / T], cx<R> *lds, const
                 });
             }
         }
-        lf_stage<N, T, SIGN, false, p.lo[d], p.sz[d]>(v, tw, t);
+        lf_stage<N, T, SIGN, false, d, PRE>(v, tw, t, pre);
     });
 }
 
 // Spectrum (layout S) -> time (layout L).
-template <int N, int T, int SIGN, int B, typename R, int DB = 0, int P0 = 0, int RS = B * N>
-__device__ __forceinline__ void lfft_dit(cx<R> (&v)[B][N / T], cx<R> *lds, const cx<R> *__restrict__ tw, int t)
+template <int N, int T, int SIGN, int B, typename R, int DB = 0, int P0 = 0, int RS = B * N, bool PRE = false>
+__device__ __forceinline__ void lfft_dit(cx<R> (&v)[B][N / T], cx<R> *lds, const cx<R> *__restrict__ tw, int t,
+                                         const LfTw<N, T, R> *pre = nullptr)
 {
     constexpr LfPlan p = LF<N, T>::p;
     static_assert(DB || P0 == 0, "one exchange region without double buffering");
     sfor<0, p.nd>([&](auto Jj) {
         constexpr int d = p.nd - 1 - decltype(Jj)::value;
-        lf_stage<N, T, SIGN, true, p.lo[d], p.sz[d]>(v, tw, t);
+        lf_stage<N, T, SIGN, true, d, PRE>(v, tw, t, pre);
         if constexpr (d > 0) {
             if constexpr (p.lds[d]) {
                 // exchanges counted in DIT order: those of stages above d come first
@@ -402,15 +429,17 @@ template <int N, int T>
 constexpr int lf_next_par(int p0) { return (LF<N, T>::p.nlds % 2 == 1) ? 1 - p0 : p0; }
 
 // Single-signal wrappers.
-template <int N, int T, int SIGN, typename R, int DB = 0, int P0 = 0, int RS = N>
-__device__ __forceinline__ void lfft_dif1(cx<R> (&v)[N / T], cx<R> *lds, const cx<R> *__restrict__ tw, int t)
+template <int N, int T, int SIGN, typename R, int DB = 0, int P0 = 0, int RS = N, bool PRE = false>
+__device__ __forceinline__ void lfft_dif1(cx<R> (&v)[N / T], cx<R> *lds, const cx<R> *__restrict__ tw, int t,
+                                          const LfTw<N, T, R> *pre = nullptr)
 {
-    lfft_dif<N, T, SIGN, 1, R, DB, P0, RS>(*reinterpret_cast<cx<R>(*)[1][N / T]>(&v), lds, tw, t);
+    lfft_dif<N, T, SIGN, 1, R, DB, P0, RS, PRE>(*reinterpret_cast<cx<R>(*)[1][N / T]>(&v), lds, tw, t, pre);
 }
-template <int N, int T, int SIGN, typename R, int DB = 0, int P0 = 0, int RS = N>
-__device__ __forceinline__ void lfft_dit1(cx<R> (&v)[N / T], cx<R> *lds, const cx<R> *__restrict__ tw, int t)
+template <int N, int T, int SIGN, typename R, int DB = 0, int P0 = 0, int RS = N, bool PRE = false>
+__device__ __forceinline__ void lfft_dit1(cx<R> (&v)[N / T], cx<R> *lds, const cx<R> *__restrict__ tw, int t,
+                                          const LfTw<N, T, R> *pre = nullptr)
 {
-    lfft_dit<N, T, SIGN, 1, R, DB, P0, RS>(*reinterpret_cast<cx<R>(*)[1][N / T]>(&v), lds, tw, t);
+    lfft_dit<N, T, SIGN, 1, R, DB, P0, RS, PRE>(*reinterpret_cast<cx<R>(*)[1][N / T]>(&v), lds, tw, t, pre);
 }
 
 // LDS position of spectral mode k for an exchange by mode index (the mirror
