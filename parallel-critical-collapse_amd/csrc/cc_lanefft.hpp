@@ -122,7 +122,7 @@ struct ic {
     static constexpr int value = V;
 };
 template <int B, int E, typename F>
-__device__ __forceinline__ void sfor(F &&f)
+__host__ __device__ __forceinline__ void sfor(F &&f)
 {
     if constexpr (B < E) {
         f(ic<B>{});
@@ -136,7 +136,7 @@ constexpr int lf_tbit(int c) { return c < LF_WAVE ? c : 6 + (c - LF_WAVE); }
 // Position (natural in-place index) of register r's element during DIF stage
 // D: thread part and register part.
 template <int N, int T, int D>
-__device__ __forceinline__ int lf_pos_t(int t)
+__host__ __device__ __forceinline__ int lf_pos_t(int t)
 {
     int x = 0;
     sfor<0, LF<N, T>::p.nb>([&](auto I) {
@@ -167,7 +167,7 @@ constexpr int lf_kbit_coord(int j)
     return -1;
 }
 template <int N, int T>
-__device__ __forceinline__ int lf_kt(int t)
+__host__ __device__ __forceinline__ int lf_kt(int t)
 {
     int k = 0;
     sfor<0, LF<N, T>::p.nb>([&](auto J) {
@@ -449,7 +449,7 @@ __device__ __forceinline__ void lfft_dit1(cx<R> (&v)[N / T], cx<R> *lds, const c
 // carry are XORed into position bits 0..LB-1 (only from mode bits >= LB, so
 // the map stays a bijection of [0, N)).
 template <int N, int T, typename R>
-__device__ __forceinline__ int lf_mirror_pos(int k)
+__host__ __device__ __forceinline__ int lf_mirror_pos(int k)
 {
     constexpr int LB = sizeof(cx<R>) >= 16 ? 3 : 4;
     int x = k;

@@ -143,8 +143,15 @@ def test_newton_steps_against_the_oracle_step(case):
             assert rel(dx, ref) < max(1e-12, 10 * cond * np.finfo(float).eps)
         else:
             r, resolved = ctx.refinement()
-            print(f"  refinement residual {r:.2e}, FP64 re-solve {resolved}")
-            assert abs(r - res) <= 0.1 * res + 1e-15 or resolved
+            # both residuals are computed in FP64 (device and host, different
+            # summation orders): each differs from the exact one by rounding,
+            # ~sqrt(n) eps (|J||dx| + |F0|) / |F0| (the probabilistic dot-product
+            # bound, times 4), which at these shapes is the size of the
+            # residuals themselves
+            gam = 4 * np.sqrt(len(F0)) * np.finfo(float).eps
+            floor = 2 * gam * np.linalg.norm(np.abs(J) @ np.abs(dx) + np.abs(F0)) / np.linalg.norm(F0)
+            print(f"  refinement residual {r:.2e}, FP64 re-solve {resolved}, rounding floor {floor:.1e}")
+            assert abs(r - res) <= 0.1 * res + floor or resolved
             assert res <= 1e-10 or resolved
         ctx.close()
     # the Up and fc blocks are determined by the system; psic's high modes
