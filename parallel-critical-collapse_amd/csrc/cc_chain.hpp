@@ -105,11 +105,13 @@ constexpr int rhs_lds() { return (1 + DB) * rhs_region<N, R, DB>(); }
 
 // Evaluate f_s = RHS(yk_s, xs_s) for both stages (ODEStepper.cpp:92-136);
 // f returns N f_s (the last forward transform's 1/N is left to the caller).
+// ims: LDS, the lamIntegrate factors 1/(m k0 N) of this thread's modes kt + T*p
+// at slots t + T*p (lf_ims: set once per launch, read without global latency).
 // bc[0], bc[1]: Re of mode M-2 of each stage state (the mode-2 fix value),
 // published by the caller before the call.  xb: rhs_lds<N, R, DB>() complex.
 template <int N, int T, typename R, typename RT = float, int DB = 0>
 __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> (&f)[2][N / T / 2],
-                                         const R (&xs)[2], const RhsConst<R> &K, const R *__restrict__ invm,
+                                         const R (&xs)[2], const RhsConst<R> &K, const R *ims,
                                          cx<R> *xb, R *bc, const cx<R> *__restrict__ tw, int t,
                                          RT *rec = nullptr)
 {
@@ -233,7 +235,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {      // lamIntegrate, lambda = 0: G = f / (-i m k0)
-        const R im = invm[kt + T * p] * (K.inv_k0 * invN);
+        const R im = ims[t + T * p];
         const cx<R> v = c[p];
         c[p] = {-(v.y * im), v.x * im};
     }
@@ -328,6 +330,17 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         }
 }
 
+// The lamIntegrate factors of rhs_pair: ims[t + T*p] = invm[k] / (k0 N) for the
+// modes k = kt + T*p of thread t (slots private to the thread: no barrier needed).
+template <int N, int T, typename R>
+__device__ __forceinline__ void lf_ims(R *ims, const R *__restrict__ invm, R inv_k0, int t)
+{
+    const int kt = lf_kt<N, T>(t);
+    const R s = inv_k0 * ((R)1 / (R)N);
+#pragma unroll
+    for (int p = 0; p < N / T; ++p) ims[t + T * p] = invm[kt + T * p] * s;
+}
+
 template <int N, typename R>
 __device__ __forceinline__ RhsConst<R> make_const(double Dim, double delta)
 {
@@ -389,15 +402,19 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
 #pragma unroll
     for (int h = 0; h < H; ++h) ys[0][t + T * h] = Yg[lf_kt<N, T>(t) + T * h];
     const RhsConst<R> K = make_const<N, R>(a.dim ? a.dim[ch] : a.Dim, a.delta[ch]);
+    __shared__ R ims[N];
+    lf_ims<N, T, R>(ims, a.invm, K.inv_k0, t);      // thread-private slots
     const double *grid = a.grid + (a.goff ? a.goff[ch] : 0);
 
     const int dir = a.dir[ch];
     int idx = a.start[ch] + dir * s_begin;
     int64_t its_sum = 0;
     int32_t st = 0;
+    // the grid values of the next step are loaded one step ahead
+    double xin = grid[idx], xout = grid[idx + dir];
     for (int step = s_begin; step < s_end; ++step, idx += dir) {
-        const double xin = grid[idx], xout = grid[idx + dir];
         const double dxd = xout - xin;
+        const double xnext = (step + 1 < s_end) ? grid[idx + 2 * dir] : 0.0;
         R xs[S], A[S][S], B[S];
         const R invN = (R)1 / (R)N;     // rhs_pair returns N f: exact power-of-two rescaling
 #pragma unroll
@@ -424,7 +441,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
         for (its = 0; its < a.maxIts; ++its) {
             cx<R> f[S][H];
             if constexpr (S == 2) {
-                rhs_pair<N, T, R, RT, DB>(yk, f, xs, K, a.invm, xb, bc, a.tw, t, rec);
+                rhs_pair<N, T, R, RT, DB>(yk, f, xs, K, ims, xb, bc, a.tw, t, rec);
             } else {
                 cx<R> yp[2][H], fp[2][H];
 #pragma unroll
@@ -433,7 +450,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
                     yp[1][h] = yk[S == 3 ? 1 : 0][h];
                 }
                 const R xp[2] = {xs[0], xs[S == 3 ? 1 : 0]};
-                rhs_pair<N, T, R, RT, DB>(yp, fp, xp, K, a.invm, xb, bc, a.tw, t);
+                rhs_pair<N, T, R, RT, DB>(yp, fp, xp, K, ims, xb, bc, a.tw, t);
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
                     f[0][h] = fp[0][h];
@@ -444,7 +461,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
                     for (int h = 0; h < H; ++h) yp[0][h] = yp[1][h] = yk[2][h];
                     const R x2[2] = {xs[2], xs[2]};
                     __syncthreads();       // behind the first evaluation's last exchange
-                    rhs_pair<N, T, R, RT, DB>(yp, fp, x2, K, a.invm, xb, bc + 4, a.tw, t);
+                    rhs_pair<N, T, R, RT, DB>(yp, fp, x2, K, ims, xb, bc + 4, a.tw, t);
 #pragma unroll
                     for (int h = 0; h < H; ++h) f[2][h] = fp[0][h];
                 }
@@ -501,6 +518,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
         }
 #pragma unroll
         for (int h = 0; h < H; ++h) ys[0][t + T * h] = ys[1][t + T * h];
+        xin = xout;
+        xout = xnext;
     }
 #pragma unroll
     for (int h = 0; h < H; ++h) Yg[lf_kt<N, T>(t) + T * h] = ys[0][t + T * h];
@@ -536,7 +555,9 @@ __global__ __launch_bounds__(T) void k_rhs_pairs(const cx<R> *Y, const double *x
     __syncthreads();
     const RhsConst<R> K = make_const<N, R>(Dim, delta[2 * pr]);
     const R xsr[2] = {(R)xs[2 * pr], (R)xs[2 * pr + 1]};
-    rhs_pair<N, T, R, R>(yk, f, xsr, K, invm, xb, bc, tw, t, rec ? rec + (size_t)pr * Rec<N>::STEP : nullptr);
+    __shared__ R ims[N];
+    lf_ims<N, T, R>(ims, invm, K.inv_k0, t);
+    rhs_pair<N, T, R, R>(yk, f, xsr, K, ims, xb, bc, tw, t, rec ? rec + (size_t)pr * Rec<N>::STEP : nullptr);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
