@@ -147,9 +147,13 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     R AU[2][H], BU[2][H], AV[2][H], BV[2][H], CF[2][H];
     cx<R> c[P];                                   // packed Coeff1 of both stages
     // (3) half-period split (814-827), IA2 coefficients (834-836), RHS products
+    // Divisions (~10 VALU instructions each in FP64) are shared where two are
+    // needed together: 1/a and 1/b as b q and a q with q = 1/(a b) (the
+    // operands are far from overflow), and the per-stage constants once.
     auto split = [&](int s, const cx<R> (&z)[2][P]) {
         const R X = xs[s];
         const R inv_x = (R)1 / X;
+        const R c3_8x = K.c3 / ((R)8 * X);
 #pragma unroll
         for (int h = 0; h < H; ++h) {
             const cx<R> a = z[0][h], b = z[0][h + H];
@@ -165,11 +169,12 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
                 rs[RC::DV] = (RT)dV;
                 rs[RC::FMX] = (RT)(F - X);
             }
-            const R C1 = -(((X + F) * U * U + (X - F) * V * V) * K.c3 / ((R)8 * X) + K.dm3);
+            const R C1 = -(((X + F) * U * U + (X - F) * V * V) * c3_8x + K.dm3);
             if (s == 0) c[h].x = C1; else c[h].y = C1;
             const R two_x = (R)2 * X;
-            const R inv1 = (R)1 / (two_x * (F + X));
-            const R inv2 = (R)1 / (two_x * (F - X));
+            const R d1 = two_x * (F + X), d2 = two_x * (F - X);
+            const R q12 = (R)1 / (d1 * d2);
+            const R inv1 = d2 * q12, inv2 = d1 * q12;
             const R FD = F * K.dm2 * (U + V);
             AU[s][h] = (FD - two_x * dU) * inv1;
             BU[s][h] = K.two_dm3 * F * U * inv1;
@@ -272,27 +277,38 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
         const cx<R> Dd = A - B;
         const cx<R> S1 = {hn * Dd.y, -(hn * Dd.x)};
         cx<R> G[2];
+        if (k == 0) {
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const cx<R> hs = s ? S1 : S0;
-            const R lam = s ? lam1 : lam0;
-            if (k == 0) {
-                G[s] = (lam != (R)0) ? cx<R>{hs.x / lam, hs.y / lam} : cx<R>{(R)0, (R)0};
-            } else if (k == N / 2) {
-                G[s] = cscale(hs, lam / (lam * lam + K.q2));
-            } else {
-                const R w = (R)wavenumber<N>(k) * K.k0;
-                const R den = lam * lam + w * w;
-                // the reference's |lambda + i w| < 1e-14 (SpectralTransformer.cpp:181) as
-                // den < 1e-28: the same test without a square root (w != 0 here, so
-                // neither form is ever true at a finite period)
-                if (den < (R)1e-28) {
-                    G[s] = {(R)0, (R)0};
-                } else {
-                    const R inv = (R)1 / den;
-                    G[s] = {(hs.x * lam - hs.y * w) * inv, (hs.y * lam + hs.x * w) * inv};
-                }
+            for (int s = 0; s < 2; ++s) {
+                const cx<R> hs = s ? S1 : S0;
+                const R lam = s ? lam1 : lam0;
+                G[s] = (lam != (R)0) ? cscale(hs, (R)1 / lam) : cx<R>{(R)0, (R)0};
             }
+        } else if (k == N / 2) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const cx<R> hs = s ? S1 : S0;
+                const R lam = s ? lam1 : lam0;
+                G[s] = cscale(hs, lam / (lam * lam + K.q2));
+            }
+        } else {
+            const R w = (R)wavenumber<N>(k) * K.k0;
+            const R den0 = lam0 * lam0 + w * w, den1 = lam1 * lam1 + w * w;
+            // the reference's |lambda + i w| < 1e-14 (SpectralTransformer.cpp:181) as
+            // den < 1e-28: the same test without a square root (w != 0 here, so
+            // neither form is ever true at a finite period)
+            const bool z0 = den0 < (R)1e-28, z1 = den1 < (R)1e-28;
+            R inv0, inv1;
+            if (!z0 && !z1) {
+                const R q = (R)1 / (den0 * den1);
+                inv0 = den1 * q;
+                inv1 = den0 * q;
+            } else {
+                inv0 = z0 ? (R)0 : (R)1 / den0;
+                inv1 = z1 ? (R)0 : (R)1 / den1;
+            }
+            G[0] = {(S0.x * lam0 - S0.y * w) * inv0, (S0.y * lam0 + S0.x * w) * inv0};
+            G[1] = {(S1.x * lam1 - S1.y * w) * inv1, (S1.y * lam1 + S1.x * w) * inv1};
         }
         c[p] = {G[0].x - G[1].y, G[0].y + G[1].x};
     }
@@ -304,9 +320,10 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     for (int p = 0; p < P; ++p) {
         const int ph = (p < H) ? p : p - H;
         const R sg = (p < H) ? (R)1 : (R)-1;
+        const R qr = (R)1 / (c[p].x * c[p].y);
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const R r = E[s][p] / (s ? c[p].y : c[p].x);
+            const R r = E[s][p] * (s ? c[p].x : c[p].y) * qr;     // E_s / c_s
             if (rec) {
                 RT *rs = rec + s * RC::STAGE + t + T * p;
                 rs[RC::R] = (RT)r;
