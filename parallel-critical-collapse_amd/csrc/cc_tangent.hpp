@@ -267,7 +267,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
     __shared__ R bc[8];
     __shared__ R red[2 * ((T / 64) > 0 ? T / 64 : 1)];
     const int t = threadIdx.x;
-    const int ch = blockIdx.x;
+    // Workgroups are dispatched in blockIdx order, and a launch of all columns
+    // has a few more workgroups than the CUs beside the base chains hold: the
+    // last ones start in a second round.  Reversed, those are the first
+    // columns (the right side's Up modes, whose chains are never the slow
+    // ones), not the high fc modes of the left side.
+    const int ch = (int)gridDim.x - 1 - (int)blockIdx.x;
     if (a.status[ch] != 0) return;
     const int dir = a.dir[ch];
     // a failed base chain leaves no record to linearise around
