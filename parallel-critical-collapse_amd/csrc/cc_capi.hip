@@ -105,6 +105,7 @@ struct cc_ctx {
     int32_t *nist = nullptr;                 // device Newton loop: rises, outcome, keep, applied
     double *refres = nullptr;                // relative residual of the last refined mixed solve
     int32_t *rhold = nullptr;                // 1: that solve is to be redone with the FP64 LU
+    int32_t *rskip = nullptr;                // 1: the refinement of the current solve has stopped
     double last_refine = 0.0;                // host copies of the last mixed solve's outcome
     int32_t last_resolve = 0;
     ColumnBuffers main;                  // base + every column (FP64 Jacobian)
@@ -595,6 +596,11 @@ cc_status lu_factor(cc_ctx *c, R *dA, int n, hipStream_t s)
     return CC_OK;
 }
 
+// Relative residual at which a further FP64 refinement sweep cannot help: the
+// residual of a refined solve settles at 2e-16 .. 1e-15 at every workload shape
+// (two sweeps after the FP32 solve), and a third changes nothing.
+constexpr double kRefineFloor = 1e-14;
+
 // Policies that solve with the FP32 LU and FP64 refinement (else the FP64 LU).
 bool fp32_solve(const cc_ctx *c) { return c->p.precision == CC_MIXED || c->p.precision == CC_FP32_FD; }
 
@@ -615,15 +621,24 @@ cc_status enqueue_solve(cc_ctx *c, int n, const double *d_J, const double *d_rhs
         CCCHK(lu_factor<float>(c, c->A32, n, s));
         HIPCHK(c, hipMemcpyAsync(d_sol, d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL((ccd::k_lu_solve<float, double>), dim3(1), dim3(1024), 0, s, c->A32, n, c->perm, d_sol);
-        for (int it = 0; it < c->p.refine_iters; ++it) {       // FP64 iterative refinement
-            hipLaunchKernelGGL(ccd::k_residual, dim3(n), dim3(256), 0, s, d_J, d_sol, d_rhs, c->r, n);
+        // FP64 iterative refinement, up to refine_iters sweeps: a sweep whose
+        // residual is already at the FP64 floor (kRefineFloor relative) ends it
+        // on the device (k_refine_stop; the kernels after it return at entry)
+        CCCHK(fill<int32_t>(c, c->rskip, 0, 1, s));
+        for (int it = 0; it < c->p.refine_iters; ++it) {
+            hipLaunchKernelGGL(ccd::k_residual, dim3(n), dim3(256), 0, s, d_J, d_sol, d_rhs, c->r, n,
+                               (const int32_t *)c->rskip);
+            hipLaunchKernelGGL(ccd::k_refine_stop, dim3(1), dim3(256), 0, s, c->r, d_rhs, n, kRefineFloor, c->rskip);
             hipLaunchKernelGGL((ccd::k_lu_solve<float, double>), dim3(1), dim3(1024), 0, s, c->A32, n, c->perm,
-                               c->r);
-            hipLaunchKernelGGL(ccd::k_axpy, dim3((n + 255) / 256), dim3(256), 0, s, d_sol, c->r, n);
+                               c->r, (const int32_t *)c->rskip);
+            hipLaunchKernelGGL(ccd::k_axpy, dim3((n + 255) / 256), dim3(256), 0, s, d_sol, c->r, n,
+                               (const int32_t *)c->rskip);
         }
         // how far the refinement got: ||rhs - J sol|| / ||rhs||, read by the host
         // after the step and by a device Newton update that must not use a bad sol
-        hipLaunchKernelGGL(ccd::k_residual, dim3(n), dim3(256), 0, s, d_J, d_sol, d_rhs, c->r, n);
+        // (after an early stop r already holds the residual of the final sol)
+        hipLaunchKernelGGL(ccd::k_residual, dim3(n), dim3(256), 0, s, d_J, d_sol, d_rhs, c->r, n,
+                           (const int32_t *)c->rskip);
         hipLaunchKernelGGL(ccd::k_refine_check, dim3(1), dim3(256), 0, s, c->r, d_rhs, c->info, n, c->p.refine_tol,
                            c->refres, c->rhold);
         HIPCHK(c, hipMemcpyAsync(c->hs.refine, c->refres, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1567,6 +1582,7 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
     ALLOC(c->nist, 4);
     ALLOC(c->refres, 1);
     ALLOC(c->rhold, 1);
+    ALLOC(c->rskip, 1);
     ALLOC(c->lu_swap, 1 + 2 * ccd::LU_SWAP_MAX);
     ALLOC(c->info, 1);
     for (ColumnBuffers *B : {&c->main, &c->base}) {

@@ -46,10 +46,40 @@ __global__ void k_negate_copy(const double *in, double *out, int n)
     if (i < n) out[i] = -in[i];
 }
 
-__global__ void k_axpy(double *x, const double *d, int n)
+// x += d; nothing when *skip (the refinement has stopped).
+__global__ void k_axpy(double *x, const double *d, int n, const int32_t *skip = nullptr)
 {
+    if (skip && skip[0]) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] += d[i];
+}
+
+// Stop of the iterative refinement: *skip = 1 once ||r|| / ||b|| <= stop (r =
+// b - A x from k_residual); the refinement's later kernels then return at
+// entry, so the number of sweeps adapts without a host round trip.  One
+// workgroup of 256 threads.
+__global__ __launch_bounds__(256) void k_refine_stop(const double *r, const double *b, int n, double stop,
+                                                     int32_t *skip)
+{
+    __shared__ double red[2][4];
+    if (skip[0]) return;
+    double sr = 0.0, sb = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        sr += r[i] * r[i];
+        sb += b[i] * b[i];
+    }
+    sr = wave_sum(sr);
+    sb = wave_sum(sb);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = sr;
+        red[1][threadIdx.x >> 6] = sb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double tr = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        const double tb = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        if (tr <= stop * stop * tb) skip[0] = 1;
+    }
 }
 
 // Relative residual of a refined solve: out[0] = ||r|| / ||b|| (r = b - A x

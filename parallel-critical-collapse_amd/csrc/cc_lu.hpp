@@ -379,9 +379,13 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const R *A, int n, const int3
     for (int r = t; r < n; r += 1024) b[r] = bs[r];
 }
 
-// r = b - A x (FP64 residual for iterative refinement), one row per thread block lane-group
-__global__ void k_residual(const double *A, const double *x, const double *b, double *r, int n)
+// r = b - A x (FP64 residual for iterative refinement), one row per thread block
+// lane-group; nothing when *skip (the refinement has stopped: r keeps the
+// residual of the current x).
+__global__ void k_residual(const double *A, const double *x, const double *b, double *r, int n,
+                           const int32_t *skip = nullptr)
 {
+    if (skip && skip[0]) return;
     const int row = blockIdx.x;
     __shared__ double red[4];
     double s = 0.0;
