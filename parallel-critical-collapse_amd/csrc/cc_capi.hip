@@ -601,6 +601,9 @@ cc_status lu_factor(cc_ctx *c, R *dA, int n, hipStream_t s)
 // (two sweeps after the FP32 solve), and a third changes nothing.
 constexpr double kRefineFloor = 1e-14;
 
+// Tangent chunks per base launch in the pipelined mixed step.
+constexpr int kBaseChunks = 2;
+
 // Policies that solve with the FP32 LU and FP64 refinement (else the FP64 LU).
 bool fp32_solve(const cc_ctx *c) { return c->p.precision == CC_MIXED || c->p.precision == CC_FP32_FD; }
 
@@ -775,17 +778,23 @@ cc_status enqueue_tangent_n(cc_ctx *c, const double *d_x)
     ab.rec = c->rec;
     ab.rec_base = c->rec_base_b;
     ccd::TangentParams<R> at = tangent_params<N, R>(c);
+    // Pipelined, the base runs BM tangent chunks per launch: each base launch
+    // on s1 waits for the previous one to end (~16 us between dependent
+    // launches), which is on the critical path while the tangent keeps up.
+    const int BM = pipe ? kBaseChunks : 1;
     for (int k = 0; k < nchunks; ++k) {
-        ab.step_begin = at.step_begin = k * S;
-        ab.step_end = at.step_end = std::min(maxsteps, (k + 1) * S);
-        CCCHK(time_mark_base(c, sb, true));
-        HIPCHK(c, Ops<N>::chain(sb, c->sched.threads_base, kb.nch, ab, t64));
-        CCCHK(time_mark_base(c, sb, false));
-        ++c->bprof_launches;
-        if (pipe) {
-            HIPCHK(c, hipEventRecord(c->chunk_ev[k], sb));
-            HIPCHK(c, hipStreamWaitEvent(s0, c->chunk_ev[k], 0));
+        at.step_begin = k * S;
+        at.step_end = std::min(maxsteps, (k + 1) * S);
+        if (k % BM == 0) {
+            ab.step_begin = k * S;
+            ab.step_end = std::min(maxsteps, (k + BM) * S);
+            CCCHK(time_mark_base(c, sb, true));
+            HIPCHK(c, Ops<N>::chain(sb, c->sched.threads_base, kb.nch, ab, t64));
+            CCCHK(time_mark_base(c, sb, false));
+            ++c->bprof_launches;
+            if (pipe) HIPCHK(c, hipEventRecord(c->chunk_ev[k / BM], sb));
         }
+        if (pipe) HIPCHK(c, hipStreamWaitEvent(s0, c->chunk_ev[k / BM], 0));
         CCCHK(time_mark(c, s0, true));
         HIPCHK(c, Ops<N>::tangent(s0, c->sched.threads_tangent, nt, at));
         CCCHK(time_mark(c, s0, false));
