@@ -86,6 +86,10 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
     constexpr int pS = DB ? lf_next_par<N, T>(pC) : 0, pG = DB ? 1 - pS : 0;
     constexpr int pH = DB ? lf_next_par<N, T>(pG) : 0;
     const int kt = lf_kt<N, T>(t);      // spectral modes of this thread: kt + T*p
+    // the stage twiddles, loaded once per evaluation and held in registers
+    constexpr bool PRE = sizeof(R) == 4;   // (the FP64 tangent has no register room)
+    LfTw<N, T, R> twr{};
+    if constexpr (PRE) twr = lf_twiddles<N, T, R>(tw, t);
     // per-stage products that survive to the pointwise tangent RHS
     // (the base-only products B_U, B_V, C_F are recomputed from the record in
     // the pointwise stage: held across the transforms they would spill)
@@ -122,8 +126,8 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             const cx<R> v = z[0][p];
             z[1][p] = {w * v.y, -(w * v.x)};
         }
-        if (s == 0) lfft_dit<N, T, -1, 2, R, DB, pA, RS>(z, xb, tw, t);
-        else lfft_dit<N, T, -1, 2, R, DB, pB, RS>(z, xb, tw, t);
+        if (s == 0) lfft_dit<N, T, -1, 2, R, DB, pA, RS, PRE>(z, xb, tw, t, &twr);
+        else lfft_dit<N, T, -1, 2, R, DB, pB, RS, PRE>(z, xb, tw, t, &twr);
         // The record of a step is the same for every fixed-point iteration and
         // every transform; an opaque offset keeps this stage's loads behind its
         // transform instead of hoisted in front of it (or of the loop), where
@@ -165,7 +169,7 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         }
     }
     // dIA2 = solveInhom with the base's integrating factor and lambda
-    lfft_dif1<N, T, +1, R, DB, pC, RS>(c, xb, tw, t);
+    lfft_dif1<N, T, +1, R, DB, pC, RS, PRE>(c, xb, tw, t, &twr);
     // (the transform's 1/N, a power of two, is folded into the separation below)
 #pragma unroll
     for (int p = 0; p < P; ++p) xb[pS * RS + lf_mirror_pos<N, T, R>(kt + T * p)] = c[p];
@@ -202,7 +206,7 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
         }
         c[p] = {G[0].x - G[1].y, G[0].y + G[1].x};
     }
-    lfft_dit1<N, T, -1, R, DB, pG, RS>(c, xb, tw, t);
+    lfft_dit1<N, T, -1, R, DB, pG, RS, PRE>(c, xb, tw, t, &twr);
     // pointwise tangent RHS: the half-period products of point jh serve both
     // points jh and jh + N/2 (U, V odd and F even under the half shift)
     cx<R> yp[2][P];
@@ -234,7 +238,7 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             }
         }
     }
-    lfft_dif<N, T, +1, 2, R, DB, pH, RS>(yp, xb, tw, t);
+    lfft_dif<N, T, +1, 2, R, DB, pH, RS, PRE>(yp, xb, tw, t, &twr);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
