@@ -517,8 +517,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
                 bc[1] = yk[S == 1 ? 0 : 1][H - 1].x;
                 if constexpr (S == 3) bc[4] = bc[5] = yk[2][H - 1].x;
             }
+            // the barrier in block_sum publishes the fix values in bc (T > 64);
+            // the next call is behind the next evaluation's exchange barriers
             const R norm2 = block_sum<T, R>(loc, red, t);
-            __syncthreads();                                  // publishes the fix values in bc
+            if constexpr (T == 64) __syncthreads();
             const double rms = sqrt((double)norm2 / (double)(N * S));
             double tol = a.precision;
             if (its > a.maxIts / 2) tol = a.precision * 10.0 * (2.0 * (double)its / (double)a.maxIts - 0.5);

@@ -294,13 +294,71 @@ __device__ __forceinline__ void fft1(cx<R> (&v)[N / T], cx<R>* lds, const cx<R>*
     fft_block<N, T, SIGN, 1, R, DB, P0, TRAIL, RS>(vb, lds, tw, t);
 }
 
+// Wave sum by register exchanges, no LDS: each of the six steps adds a lane's
+// value and a partner's (DPP quad permutes 0xB1 = lane ^ 1 and 0x4E = lane ^ 2,
+// row_half_mirror 0x141 = lane ^ 7 within 8 lanes, row_mirror 0x140 = lane ^ 15
+// within 16, then v_permlane16_swap and v_permlane32_swap across rows).  After
+// each step all lanes of a group hold the same bits, and partners add the same
+// two operands (IEEE addition commutes), so every lane returns the same bits:
+// a loop exit taken on the sum is uniform across the workgroup.
+template <int CTRL, typename A>
+__device__ __forceinline__ A wave_add_dpp(A v)
+{
+    constexpr int W = (int)(sizeof(A) / 4);
+    uint32_t u[W], p[W];
+    __builtin_memcpy(u, &v, sizeof(A));
+#pragma unroll
+    for (int w = 0; w < W; ++w) p[w] = (uint32_t)__builtin_amdgcn_mov_dpp((int)u[w], CTRL, 0xF, 0xF, true);
+    A o;
+    __builtin_memcpy(&o, p, sizeof(A));
+    return v + o;
+}
+
+template <int C, typename A>
+__device__ __forceinline__ A wave_add_swap(A v)
+{
+    constexpr int W = (int)(sizeof(A) / 4);
+    uint32_t u[W], a[W], b[W];
+    __builtin_memcpy(u, &v, sizeof(A));
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        if constexpr (C == 4) {
+            const auto r = __builtin_amdgcn_permlane16_swap(u[w], u[w], false, false);
+            a[w] = r[0];
+            b[w] = r[1];
+        } else {
+            const auto r = __builtin_amdgcn_permlane32_swap(u[w], u[w], false, false);
+            a[w] = r[0];
+            b[w] = r[1];
+        }
+    }
+    // one of the two holds this lane's row value, the other the partner row's
+    A x, y;
+    __builtin_memcpy(&x, a, sizeof(A));
+    __builtin_memcpy(&y, b, sizeof(A));
+    return x + y;
+}
+
+template <typename A>
+__device__ __forceinline__ A wave_sum(A v)
+{
+    v = wave_add_dpp<0xB1>(v);
+    v = wave_add_dpp<0x4E>(v);
+    v = wave_add_dpp<0x141>(v);
+    v = wave_add_dpp<0x140>(v);
+    v = wave_add_swap<4>(v);
+    return wave_add_swap<5>(v);
+}
+
 // Workgroup sum of one value per thread (T threads, T a multiple of 64).
-// red: LDS scratch of at least T/64 values.  Returns the sum to every thread.
+// red: LDS scratch of at least T/64 values.  Returns the same bits to every
+// thread.  For T > 64 the one barrier inside also publishes the caller's
+// earlier LDS writes; red is reused without a trailing barrier, so two calls
+// must be separated by another barrier of the caller's.
 template <int T, typename A>
 __device__ __forceinline__ A block_sum(A v, A* red, int t)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    v = wave_sum(v);
     if constexpr (T == 64) {
         return v;
     } else {
@@ -309,21 +367,18 @@ __device__ __forceinline__ A block_sum(A v, A* red, int t)
         A s = red[0];
 #pragma unroll
         for (int w = 1; w < T / 64; ++w) s += red[w];
-        __syncthreads();
         return s;
     }
 }
 
 // Workgroup sums of two values per thread in one pass (one LDS round trip).
-// red: LDS scratch of at least 2*T/64 values.
+// red: LDS scratch of at least 2*T/64 values.  Same barrier contract as
+// block_sum.
 template <int T, typename A>
 __device__ __forceinline__ void block_sum2(A &a, A &b, A* red, int t)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_xor(a, off, 64);
-        b += __shfl_xor(b, off, 64);
-    }
+    a = wave_sum(a);
+    b = wave_sum(b);
     if constexpr (T > 64) {
         if ((t & 63) == 0) {
             red[t >> 6] = a;
@@ -336,7 +391,6 @@ __device__ __forceinline__ void block_sum2(A &a, A &b, A* red, int t)
             sa += red[w];
             sb += red[T / 64 + w];
         }
-        __syncthreads();
         a = sa;
         b = sb;
     }

@@ -33,14 +33,6 @@ constexpr int lu_nb()
     return sizeof(R) == 8 ? 16 : 32;
 }
 
-template <typename R>
-__device__ __forceinline__ R wave_sum(R v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 // Value of x in lane l (l uniform across the wave).
 __device__ __forceinline__ float lane_value(float x, int l)
 {

@@ -385,7 +385,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
                 bc[1] = dk[1][H - 1].x;
             }
             R d2 = loc, m2 = mag;
-            block_sum2<T, R>(d2, m2, red, t);          // its barriers publish bc[0..1] (T > 64)
+            block_sum2<T, R>(d2, m2, red, t);          // its barrier publishes bc[0..1] (T > 64)
             if constexpr (T == 64) __syncthreads();
             // linear fixed point: stop when the update is below rtol of the
             // iterate, relaxed late as the base iteration's tolerance
