@@ -91,6 +91,32 @@ struct RhsConst {
     R dm2, dm3, c3, two_dm3, mg, q2;  // D-2, D-3, (D-2)^3, 2(D-3), -(D-3), (k0 N/2)^2
 };
 
+// A stage's x and the split's x-dependent factors, formed once per IRK step
+// (the fixed-point iterations of a step share them).
+template <typename R>
+struct StageX {
+    R X, inv_x, c3_8x;                // x, 1/x, (D-2)^3 / (8 x)
+};
+// (x is the same in every lane: the values are moved to scalar registers,
+// where they stay across the fixed-point loop instead of being recomputed
+// in it to save vector registers)
+template <typename R>
+__device__ __forceinline__ R lane_uniform(R v)
+{
+    constexpr int W = (int)(sizeof(R) / 4);
+    uint32_t u[W];
+    __builtin_memcpy(u, &v, sizeof(R));
+#pragma unroll
+    for (int w = 0; w < W; ++w) u[w] = (uint32_t)__builtin_amdgcn_readfirstlane((int)u[w]);
+    __builtin_memcpy(&v, u, sizeof(R));
+    return v;
+}
+template <typename R>
+__device__ __forceinline__ StageX<R> stage_x(R X, const RhsConst<R> &K)
+{
+    return {lane_uniform(X), lane_uniform((R)1 / X), lane_uniform(K.c3 / ((R)8 * X))};
+}
+
 // DB = 1 evaluations run the two stages' first inverse transforms as one
 // batch of four signals (one set of exchanges and barriers instead of two; the
 // butterflies of each signal are unchanged, so are the results).  Their LDS
@@ -111,7 +137,7 @@ constexpr int rhs_lds() { return (1 + DB) * rhs_region<N, R, DB>(); }
 // published by the caller before the call.  xb: rhs_lds<N, R, DB>() complex.
 template <int N, int T, typename R, typename RT = float, int DB = 0>
 __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> (&f)[2][N / T / 2],
-                                         const R (&xs)[2], const RhsConst<R> &K, const R *ims,
+                                         const StageX<R> (&sx)[2], const RhsConst<R> &K, const R *ims,
                                          cx<R> *xb, R *bc, const cx<R> *__restrict__ tw, int t,
                                          RT *rec = nullptr)
 {
@@ -151,9 +177,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     // needed together: 1/a and 1/b as b q and a q with q = 1/(a b) (the
     // operands are far from overflow), and the per-stage constants once.
     auto split = [&](int s, const cx<R> (&z)[2][P]) {
-        const R X = xs[s];
-        const R inv_x = (R)1 / X;
-        const R c3_8x = K.c3 / ((R)8 * X);
+        const R X = sx[s].X, inv_x = sx[s].inv_x, c3_8x = sx[s].c3_8x;
 #pragma unroll
         for (int h = 0; h < H; ++h) {
             const cx<R> a = z[0][h], b = z[0][h + H];
@@ -173,7 +197,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
             if (s == 0) c[h].x = C1; else c[h].y = C1;
             const R two_x = (R)2 * X;
             const R d1 = two_x * (F + X), d2 = two_x * (F - X);
-            const R q12 = (R)1 / (d1 * d2);
+            const R q12 = rcp_fast(d1 * d2);
             const R inv1 = d2 * q12, inv2 = d1 * q12;
             const R FD = F * K.dm2 * (U + V);
             AU[s][h] = (FD - two_x * dU) * inv1;
@@ -300,7 +324,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
             const bool z0 = den0 < (R)1e-28, z1 = den1 < (R)1e-28;
             R inv0, inv1;
             if (!z0 && !z1) {
-                const R q = (R)1 / (den0 * den1);
+                const R q = rcp_fast(den0 * den1);
                 inv0 = den1 * q;
                 inv1 = den0 * q;
             } else {
@@ -320,7 +344,7 @@ __device__ __forceinline__ void rhs_pair(const cx<R> (&yk)[2][N / T / 2], cx<R> 
     for (int p = 0; p < P; ++p) {
         const int ph = (p < H) ? p : p - H;
         const R sg = (p < H) ? (R)1 : (R)-1;
-        const R qr = (R)1 / (c[p].x * c[p].y);
+        const R qr = rcp_fast(c[p].x * c[p].y);
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const R r = E[s][p] * (s ? c[p].x : c[p].y) * qr;     // E_s / c_s
@@ -432,11 +456,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
     for (int step = s_begin; step < s_end; ++step, idx += dir) {
         const double dxd = xout - xin;
         const double xnext = (step + 1 < s_end) ? grid[idx + 2 * dir] : 0.0;
-        R xs[S], A[S][S], B[S];
+        StageX<R> xs[S];
+        R A[S][S], B[S];
         const R invN = (R)1 / (R)N;     // rhs_pair returns N f: exact power-of-two rescaling
 #pragma unroll
         for (int i = 0; i < S; ++i) {
-            xs[i] = (R)(xin + dxd * a.c[i]);
+            xs[i] = stage_x((R)(xin + dxd * a.c[i]), K);
             B[i] = (R)(dxd * a.b[i]) * invN;
 #pragma unroll
             for (int k = 0; k < S; ++k) A[i][k] = (R)(dxd * a.a[i][k]) * invN;
@@ -466,7 +491,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
                     yp[0][h] = yk[0][h];
                     yp[1][h] = yk[S == 3 ? 1 : 0][h];
                 }
-                const R xp[2] = {xs[0], xs[S == 3 ? 1 : 0]};
+                const StageX<R> xp[2] = {xs[0], xs[S == 3 ? 1 : 0]};
                 rhs_pair<N, T, R, RT, DB>(yp, fp, xp, K, ims, xb, bc, a.tw, t);
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
@@ -476,7 +501,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(chain_waves<N
                 if constexpr (S == 3) {
 #pragma unroll
                     for (int h = 0; h < H; ++h) yp[0][h] = yp[1][h] = yk[2][h];
-                    const R x2[2] = {xs[2], xs[2]};
+                    const StageX<R> x2[2] = {xs[2], xs[2]};
                     __syncthreads();       // behind the first evaluation's last exchange
                     rhs_pair<N, T, R, RT, DB>(yp, fp, x2, K, ims, xb, bc + 4, a.tw, t);
 #pragma unroll
@@ -573,7 +598,7 @@ __global__ __launch_bounds__(T) void k_rhs_pairs(const cx<R> *Y, const double *x
     }
     __syncthreads();
     const RhsConst<R> K = make_const<N, R>(Dim, delta[2 * pr]);
-    const R xsr[2] = {(R)xs[2 * pr], (R)xs[2 * pr + 1]};
+    const StageX<R> xsr[2] = {stage_x((R)xs[2 * pr], K), stage_x((R)xs[2 * pr + 1], K)};
     __shared__ R ims[N];
     lf_ims<N, T, R>(ims, invm, K.inv_k0, t);
     rhs_pair<N, T, R, R>(yk, f, xsr, K, ims, xb, bc, tw, t, rec ? rec + (size_t)pr * Rec<N>::STEP : nullptr);

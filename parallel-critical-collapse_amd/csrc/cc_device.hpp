@@ -294,6 +294,24 @@ __device__ __forceinline__ void fft1(cx<R> (&v)[N / T], cx<R>* lds, const cx<R>*
     fft_block<N, T, SIGN, 1, R, DB, P0, TRAIL, RS>(vb, lds, tw, t);
 }
 
+// 1/x for finite x of moderate exponent (no denormal, infinite or
+// near-overflow operands): FP64 as v_rcp_f64 refined by two Newton steps,
+// within an ulp of the quotient (the IEEE division adds operand scaling and a
+// final correction: 11 instructions against 5); FP32 as v_rcp_f32 (1 ulp).
+template <typename R>
+__device__ __forceinline__ R rcp_fast(R x)
+{
+    if constexpr (sizeof(R) == 4) {
+        return __builtin_amdgcn_rcpf(x);
+    } else {
+        R r = __builtin_amdgcn_rcp(x);
+        R e = __builtin_fma(-x, r, (R)1);
+        r = __builtin_fma(r, e, r);
+        e = __builtin_fma(-x, r, (R)1);
+        return __builtin_fma(r, e, r);
+    }
+}
+
 // Wave sum by register exchanges, no LDS: each of the six steps adds a lane's
 // value and a partner's (DPP quad permutes 0xB1 = lane ^ 1 and 0x4E = lane ^ 2,
 // row_half_mirror 0x141 = lane ^ 7 within 8 lanes, row_mirror 0x140 = lane ^ 15
