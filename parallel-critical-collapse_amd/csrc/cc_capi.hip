@@ -84,8 +84,8 @@ struct StatusBlock {
 struct cc_ctx {
     cc_problem p{};
     int N = 0, M = 0, n = 0, X = 0, iM = 0, iR = 0;
-    hipStream_t s0 = nullptr, s1 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t s0 = nullptr, s1 = nullptr, s2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
     std::vector<hipEvent_t> chunk_ev;    // base chunk k done (pipelined mixed step)
     std::string err;
     int64_t counters[3] = {0, 0, 0};
@@ -121,6 +121,12 @@ struct cc_ctx {
     int32_t *t_cols = nullptr, *t_lcols = nullptr, *t_rcols = nullptr, *t_lch = nullptr, *t_rch = nullptr;
     int32_t *t_lidx = nullptr, *t_ridx = nullptr;
     int64_t *t_its = nullptr;
+    // launch order of the tangent chains: the first nt - t_nsolo in the main
+    // launch, the last t_nsolo (the slowest of the previous Jacobian) alone on
+    // their CUs in a launch of their own on s2
+    int32_t *t_order = nullptr;
+    std::vector<int32_t> t_order_h;
+    int t_nsolo = 0, t_nsolo_used = 0;
     std::vector<int32_t> t_cols_h, t_nsteps_h;
     int t_nl = 0, t_nr = 0;
     double tangent_rtol = 1e-6, tangent_h = 1e-5;   // CC_FP64_TANGENT: rtol 1e-13 (1e-10 at Ntau 2048)
@@ -756,6 +762,21 @@ cc_status tangent_prologue(cc_ctx *c, const double *d_x, hipStream_t s)
     return CC_OK;
 }
 
+// LDS a solo tangent launch adds to the kernel's own: more than half a CU's
+// 160 KB in all, so no second tangent workgroup fits beside it.
+constexpr size_t kSoloLds = 64 * 1024;
+
+template <int N>
+hipError_t launch_solo(cc_ctx *c, hipStream_t s, int nsolo, const ccd::TangentParams<float> &a)
+{
+    return Ops<N>::tangent(s, c->sched.threads_tangent, nsolo, a, kSoloLds);
+}
+template <int N>
+hipError_t launch_solo(cc_ctx *, hipStream_t, int, const ccd::TangentParams<double> &)
+{
+    return hipErrorInvalidValue;                 // FP64 tangents never split
+}
+
 template <int N, typename R>
 cc_status enqueue_tangent_n(cc_ctx *c, const double *d_x)
 {
@@ -778,6 +799,21 @@ cc_status enqueue_tangent_n(cc_ctx *c, const double *d_x)
     ab.rec = c->rec;
     ab.rec_base = c->rec_base_b;
     ccd::TangentParams<R> at = tangent_params<N, R>(c);
+    // The slowest chains of the previous Jacobian (t_nsolo, FP32 pipelined
+    // only) run in a launch of their own on s2, each workgroup alone on its CU
+    // (LDS padded past half a CU): sharing a CU with two other workgroups for
+    // the first part of every launch slows them, and every launch they are in
+    // lasts as long as they do.  The main launch takes the other chains.
+    const int nsolo = (pipe && !t64 && c->sched.threads_tangent == ccd::default_threads(N)) ? c->t_nsolo : 0;
+    c->t_nsolo_used = nsolo;
+    const int nmain = nt - nsolo;
+    ccd::TangentParams<R> as = at;
+    if (nsolo) {
+        at.order = c->t_order;
+        as.order = c->t_order + nmain;
+        HIPCHK(c, hipEventRecord(c->ev_fork2, s0));
+        HIPCHK(c, hipStreamWaitEvent(c->s2, c->ev_fork2, 0));
+    }
     // Pipelined, the base runs BM tangent chunks per launch: each base launch
     // on s1 waits for the previous one to end (~16 us between dependent
     // launches), which is on the critical path while the tangent keeps up.
@@ -796,9 +832,19 @@ cc_status enqueue_tangent_n(cc_ctx *c, const double *d_x)
         }
         if (pipe) HIPCHK(c, hipStreamWaitEvent(s0, c->chunk_ev[k / BM], 0));
         CCCHK(time_mark(c, s0, true));
-        HIPCHK(c, Ops<N>::tangent(s0, c->sched.threads_tangent, nt, at));
+        HIPCHK(c, Ops<N>::tangent(s0, c->sched.threads_tangent, nmain, at));
         CCCHK(time_mark(c, s0, false));
         ++c->prof_launches;
+        if (nsolo) {
+            as.step_begin = at.step_begin;
+            as.step_end = at.step_end;
+            HIPCHK(c, hipStreamWaitEvent(c->s2, c->chunk_ev[k / BM], 0));
+            HIPCHK(c, launch_solo<N>(c, c->s2, nsolo, as));
+        }
+    }
+    if (nsolo) {
+        HIPCHK(c, hipEventRecord(c->ev_join2, c->s2));
+        HIPCHK(c, hipStreamWaitEvent(s0, c->ev_join2, 0));
     }
     HIPCHK(c, Ops<N>::mismatch(sb, false, kb.ncols, c->base.Y, c->base.lidx, c->base.ridx, c->base.delta,
                                c->p.PrecisionIRK, c->tw64, c->base.F, c->base.mstatus));
@@ -829,23 +875,67 @@ cc_status enqueue_tangent(cc_ctx *c, const double *d_x)
 
 bool tangent_policy(const cc_ctx *c) { return c->p.precision == CC_MIXED || c->p.precision == CC_FP64_TANGENT; }
 
+// The chains of the next FP32 Jacobian that get a CU each (t_nsolo, at most
+// kMaxSolo): those that needed at least twice the median number of
+// fixed-point evaluations in this one, the slowest first.  The order array
+// puts them last; a change of the set re-captures a step graph (its launch
+// sizes are part of the graph).
+constexpr int kMaxSolo = 16;
+
+void drop_graph(cc_ctx *c);
+
+cc_status pick_solo(cc_ctx *c)
+{
+    const int nt = (int)c->t_cols_h.size();
+    if (c->p.precision != CC_MIXED || nt < 4 * kMaxSolo) return CC_OK;
+    std::vector<int64_t> v(c->hs.tits, c->hs.tits + nt);
+    std::vector<int64_t> sorted = v;
+    std::nth_element(sorted.begin(), sorted.begin() + nt / 2, sorted.end());
+    const int64_t med = sorted[nt / 2];
+    std::vector<int32_t> slow;
+    for (int b = 0; b < nt; ++b)
+        if (med > 0 && v[b] >= 2 * med) slow.push_back(b);
+    std::sort(slow.begin(), slow.end(), [&](int32_t x, int32_t y) { return v[x] > v[y] || (v[x] == v[y] && x < y); });
+    if ((int)slow.size() > kMaxSolo) slow.resize(kMaxSolo);
+    std::sort(slow.begin(), slow.end());
+    std::vector<char> is_slow(nt, 0);
+    for (int32_t b : slow) is_slow[b] = 1;
+    std::vector<int32_t> order;
+    order.reserve(nt);
+    for (int b = 0; b < nt; ++b)
+        if (!is_slow[b]) order.push_back(b);
+    for (int32_t b : slow) order.push_back(b);
+    if (order == c->t_order_h && (int)slow.size() == c->t_nsolo) return CC_OK;
+    c->t_order_h = order;
+    c->t_nsolo = (int)slow.size();
+    HIPCHK(c, up(c->t_order_h, c->t_order));
+    drop_graph(c);
+    return CC_OK;
+}
+
 // Statuses and counters of a tangent-linear Jacobian (after the stream synchronised).
 cc_status eval_tangent(cc_ctx *c, int32_t *col_status)
 {
     const cc_status sb = eval_columns(c, c->base_plan, counts_of(c->base_plan), true, nullptr, true);
     const int nt = (int)c->t_cols_h.size();
-    int64_t itsum = 0, steps = 0;
+    int64_t itsum = 0, steps = 0, its_main = 0, steps_main = 0;
     bool bad = false;
     for (int b = 0; b < nt; ++b) {
         itsum += c->hs.tits[b];
         steps += c->t_nsteps_h[b];
         if (c->hs.tstatus[b]) bad = true;
     }
+    // the timed launches are the main ones: their chains' counts
+    for (int i = 0; i < nt - c->t_nsolo_used; ++i) {
+        its_main += c->hs.tits[c->t_order_h[i]];
+        steps_main += c->t_nsteps_h[c->t_order_h[i]];
+    }
     c->counters[0] += steps;
     c->counters[1] += itsum;
     c->counters[2] += nt;
-    c->prof_its = itsum;
-    c->prof_steps = steps;
+    c->prof_its = its_main;
+    c->prof_steps = steps_main;
+    if (!bad) CCCHK(pick_solo(c));
     c->bprof_its = c->bprof_steps = 0;
     const Counts kb = counts_of(c->base_plan);
     for (int i = 0; i < kb.nch; ++i) {
@@ -1528,17 +1618,20 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
         fprintf(stderr, "cc_ctx_create: %s: %s\n", w, hipGetErrorString(e));
         for (void *q : c->allocs) (void)hipFree(q);
         if (c->hblock) (void)hipHostFree(c->hblock);
-        for (hipEvent_t ev : {c->ev_fork, c->ev_join, c->tk0, c->tk1})
+        for (hipEvent_t ev : {c->ev_fork, c->ev_join, c->ev_fork2, c->ev_join2, c->tk0, c->tk1})
             if (ev) (void)hipEventDestroy(ev);
-        for (hipStream_t st : {c->s0, c->s1})
+        for (hipStream_t st : {c->s0, c->s1, c->s2})
             if (st) (void)hipStreamDestroy(st);
         delete c;
         return true;
     };
     if (bad(hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking), "stream")) return CC_DEVICE;
     if (bad(hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking), "stream")) return CC_DEVICE;
+    if (bad(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking), "stream")) return CC_DEVICE;
     if (bad(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event")) return CC_DEVICE;
     if (bad(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "event")) return CC_DEVICE;
+    if (bad(hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming), "event")) return CC_DEVICE;
+    if (bad(hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming), "event")) return CC_DEVICE;
     if (bad(hipEventCreate(&c->tk0), "event")) return CC_DEVICE;
     if (bad(hipEventCreate(&c->tk1), "event")) return CC_DEVICE;
     const int n = c->n, M = c->M;
@@ -1624,6 +1717,7 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
                         &c->t_rcols, &c->t_lch, &c->t_rch, &c->t_lidx, &c->t_ridx})
         ALLOC(*q, n);
     ALLOC(c->t_its, n);
+    ALLOC(c->t_order, n);
 #undef ALLOC
     if (bad(e, "hipMalloc")) return CC_DEVICE;
     // pinned status block
@@ -1662,6 +1756,9 @@ static cc_status create_ctx(const cc_problem *p, cc_ctx **out, bool own_tangent)
                         std::make_pair(&c->t_nsteps_h, c->t_nsteps), std::make_pair(&t_dir, c->t_dir)})
             if (eu == hipSuccess) eu = up(*pr.first, pr.second);
         if (eu == hipSuccess) eu = up(rbt, c->rec_base_t);
+        c->t_order_h.resize(c->t_cols_h.size());
+        for (size_t b = 0; b < c->t_order_h.size(); ++b) c->t_order_h[b] = (int32_t)b;
+        if (eu == hipSuccess && !c->t_order_h.empty()) eu = up(c->t_order_h, c->t_order);
         if (bad(eu, "plan")) return CC_DEVICE;
     }
     // x-grid, NewtonSolver.cpp:477-519
@@ -1718,15 +1815,17 @@ cc_status cc_ctx_destroy(cc_ctx *c)
     if (!c) return CC_BADARG;
     (void)hipStreamSynchronize(c->s0);
     (void)hipStreamSynchronize(c->s1);
+    (void)hipStreamSynchronize(c->s2);
     drop_graph(c);
     for (void *q : c->allocs) (void)hipFree(q);
     if (c->hblock) (void)hipHostFree(c->hblock);
     for (hipEvent_t e : c->chunk_ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->bev) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {c->ev_fork, c->ev_join, c->tk0, c->tk1}) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_fork2, c->ev_join2, c->tk0, c->tk1}) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->s0);
     (void)hipStreamDestroy(c->s1);
+    (void)hipStreamDestroy(c->s2);
     delete c;
     return CC_OK;
 }

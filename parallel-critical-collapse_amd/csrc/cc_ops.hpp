@@ -44,7 +44,11 @@ struct Ops {
     static hipError_t boundary_tangent(hipStream_t s, bool left, bool fp64, int ncols, const double *x,
                                        const int32_t *cols, const int32_t *chain, double hrel, double Xb,
                                        double Dim, const cx<double> *tw, void *dY);
-    static hipError_t tangent(hipStream_t s, int T, int nchains, const TangentParams<float> &a);
+    // solo_lds > 0: the k_tangent<..., SOLO> launch (kThreads) with that much
+    // dynamic LDS on top of the kernel's own, so no other workgroup fits beside
+    // each of its workgroups
+    static hipError_t tangent(hipStream_t s, int T, int nchains, const TangentParams<float> &a,
+                              size_t solo_lds = 0);
     static hipError_t tangent(hipStream_t s, int T, int nchains, const TangentParams<double> &a);  // kThreads only
     static hipError_t tangent_column(hipStream_t s, bool fp64, int ncols, const void *dY, const int32_t *cols,
                                      const int32_t *lidx, const int32_t *ridx, const cx<double> *tw, double *J);

@@ -129,8 +129,14 @@ hipError_t Ops<N>::boundary_tangent(hipStream_t s, bool left, bool fp64, int nco
 }
 
 template <int N>
-hipError_t Ops<N>::tangent(hipStream_t s, int T, int nchains, const TangentParams<float> &a)
+hipError_t Ops<N>::tangent(hipStream_t s, int T, int nchains, const TangentParams<float> &a, size_t solo_lds)
 {
+    if (solo_lds > 0) {
+        if (T != kThreads) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_tangent<N, kThreads, float, true>), dim3((unsigned)nchains), dim3(kThreads), solo_lds,
+                           s, a);
+        return hipGetLastError();
+    }
     if (T == kThreads) {
         hipLaunchKernelGGL((k_tangent<N, kThreads, float>), dim3((unsigned)nchains), dim3(kThreads), 0, s, a);
     } else {

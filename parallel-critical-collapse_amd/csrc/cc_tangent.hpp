@@ -53,6 +53,7 @@ struct TangentParams {
     int32_t step_begin, step_end, maxIts;
     double Dim, rtol;         // relative RMS tolerance of the tangent fixed point
     double a00, a01, a10, a11, b0, b1, c0, c1;
+    const int32_t *order;     // launch slot -> chain (part of a permutation of the chains), null: identity
 };
 
 // FP32 tangents up to N = 1024 double-buffer their transform exchanges (one
@@ -258,7 +259,9 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
 template <int N, int T, typename R>
 constexpr int tangent_waves() { return sizeof(R) == 8 ? (N / T >= 8 ? 1 : 2) : (N / T >= 8 ? 2 : 3); }
 
-template <int N, int T, typename R>
+// SOLO: the launch of the chains that run alone on their CUs (the host pads
+// the launch's LDS); the code is the same, the name tells the launches apart.
+template <int N, int T, typename R, bool SOLO = false>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves<N, T, R>()))) void k_tangent(TangentParams<R> a)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
@@ -276,7 +279,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves
     // last ones start in a second round.  Reversed, those are the first
     // columns (the right side's Up modes, whose chains are never the slow
     // ones), not the high fc modes of the left side.
-    const int ch = (int)gridDim.x - 1 - (int)blockIdx.x;
+    const int slot = (int)gridDim.x - 1 - (int)blockIdx.x;
+    const int ch = a.order ? a.order[slot] : slot;
     if (a.status[ch] != 0) return;
     const int dir = a.dir[ch];
     // a failed base chain leaves no record to linearise around

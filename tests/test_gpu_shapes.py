@@ -159,3 +159,26 @@ def test_newton_steps_against_the_oracle_step(case):
     # Ntau=512: Up 1e-3, fc 1.4e-3 (FP64) and 2.9e-3 (mixed), psic 1.4)
     for name, (tot, errs, _) in out.items():
         assert errs["Up"] < 1e-2 and errs["fc"] < 1e-2, (name, errs)
+
+
+def test_slow_tangent_chains_run_alone(case):
+    """After an FP32 tangent Jacobian, the chains that needed at least twice
+    the median number of fixed-point evaluations (at most 16) run in a launch
+    of their own in the next one, each workgroup alone on its CU: the same
+    columns bit for bit.  The timed (main) launches then cover fewer chain
+    steps; at the workload shape (Ntau=1024) the split is active."""
+    s = case["s"]
+    sh = s["shape"]
+    ctx = ctx_for(s, precision=cc.CC_MIXED)
+    ctx.set_schedule(pipeline=1, steps_per_launch=12)
+    ctx.profile(True)
+    J0, cs0, st0 = ctx.jacobian(s["x"], case["F0o"])
+    steps0 = ctx.chain_timing()[3]
+    J1, cs1, st1 = ctx.jacobian(s["x"], case["F0o"])
+    steps1 = ctx.chain_timing()[3]
+    assert st0 == 0 and st1 == 0 and not cs0.any() and not cs1.any()
+    print(f"Ntau={sh.Ntau}: chain steps in the main launches {steps0} -> {steps1}")
+    assert np.array_equal(J0, J1)
+    assert steps1 <= steps0
+    if sh.Ntau == 1024:
+        assert steps1 < steps0
