@@ -184,7 +184,10 @@ cc_status cc_counters(cc_ctx *ctx, int64_t out[3]);
  * cc_chain_timing reports the last launch set: elapsed ms, number of kernel
  * launches, fixed-point iterations and IRK steps they executed.  enable: 0
  * off; 1 those launches and the base chains' (cc_base_timing); 2 those
- * launches only, no events on the base chains' stream. */
+ * launches only, no events on the base chains' stream.  In the mixed policy
+ * the slowest tangent chains of the previous Jacobian run in a launch of their
+ * own on a third stream; those launches are not timed, and their iterations
+ * and steps are not in the counts. */
 cc_status cc_profile(cc_ctx *ctx, int32_t enable);
 cc_status cc_chain_timing(cc_ctx *ctx, double *ms, int32_t *launches, int64_t *its, int64_t *steps);
 /* The same for the FP64 base chains of the tangent-linear policies (the base
