@@ -368,6 +368,72 @@ __device__ __forceinline__ A wave_sum(A v)
     return wave_add_swap<5>(v);
 }
 
+// Arg-max over the wave by the same exchange pattern as wave_sum: the larger
+// value wins, the lower index on ties (the first row of the largest |a|, as
+// LAPACK's i?amax).  The rule is a strict order on (value, index), so both
+// lanes of a pair keep the same winner and every lane returns the same pair.
+// Values must not be NaN (callers start from -1 and take only v > best).
+template <typename A>
+__device__ __forceinline__ void argmax_pick(A &v, int &i, A ov, int oi)
+{
+    if (ov > v || (ov == v && oi < i)) {
+        v = ov;
+        i = oi;
+    }
+}
+template <int CTRL, typename A>
+__device__ __forceinline__ void argmax_dpp(A &v, int &i)
+{
+    constexpr int W = (int)(sizeof(A) / 4);
+    uint32_t u[W];
+    __builtin_memcpy(u, &v, sizeof(A));
+#pragma unroll
+    for (int w = 0; w < W; ++w) u[w] = (uint32_t)__builtin_amdgcn_mov_dpp((int)u[w], CTRL, 0xF, 0xF, true);
+    A ov;
+    __builtin_memcpy(&ov, u, sizeof(A));
+    const int oi = __builtin_amdgcn_mov_dpp(i, CTRL, 0xF, 0xF, true);
+    argmax_pick(v, i, ov, oi);
+}
+template <int C, typename A>
+__device__ __forceinline__ void argmax_swap(A &v, int &i)
+{
+    constexpr int W = (int)(sizeof(A) / 4);
+    uint32_t u[W], a[W], b[W];
+    __builtin_memcpy(u, &v, sizeof(A));
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        if constexpr (C == 4) {
+            const auto r = __builtin_amdgcn_permlane16_swap(u[w], u[w], false, false);
+            a[w] = r[0];
+            b[w] = r[1];
+        } else {
+            const auto r = __builtin_amdgcn_permlane32_swap(u[w], u[w], false, false);
+            a[w] = r[0];
+            b[w] = r[1];
+        }
+    }
+    const uint32_t ui = (uint32_t)i;
+    const auto ri = C == 4 ? __builtin_amdgcn_permlane16_swap(ui, ui, false, false)
+                           : __builtin_amdgcn_permlane32_swap(ui, ui, false, false);
+    // the two candidates are this lane's row value and the partner row's
+    A x, y;
+    __builtin_memcpy(&x, a, sizeof(A));
+    __builtin_memcpy(&y, b, sizeof(A));
+    v = x;
+    i = (int)ri[0];
+    argmax_pick(v, i, y, (int)ri[1]);
+}
+template <typename A>
+__device__ __forceinline__ void wave_argmax(A &v, int &i)
+{
+    argmax_dpp<0xB1>(v, i);
+    argmax_dpp<0x4E>(v, i);
+    argmax_dpp<0x141>(v, i);
+    argmax_dpp<0x140>(v, i);
+    argmax_swap<4>(v, i);
+    argmax_swap<5>(v, i);
+}
+
 // Workgroup sum of one value per thread (T threads, T a multiple of 64).
 // red: LDS scratch of at least T/64 values.  Returns the same bits to every
 // thread.  For T > 64 the one barrier inside also publishes the caller's

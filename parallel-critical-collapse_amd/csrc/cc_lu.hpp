@@ -91,42 +91,31 @@ __global__ __launch_bounds__(LU_PANEL_T) void k_lu_panel(R *A, int n, int k0, in
                 bi = i;
             }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const R ov = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ov > best || (ov == best && oi < bi)) {
-                best = ov;
-                bi = oi;
-            }
-        }
+        wave_argmax(best, bi);
         if ((t & 63) == 0) {
             sval[t >> 6] = best;
             sidx[t >> 6] = bi;
         }
         __syncthreads();
+        // every thread reduces the W wave results the same way: no serial
+        // step and no second barrier before the swap
+        R b = sval[0];
+        int i0 = sidx[0];
+#pragma unroll
+        for (int v = 1; v < W; ++v) argmax_pick(b, i0, sval[v], sidx[v]);
+        // a column with no comparable entry (all NaN) yields no candidate:
+        // keep the diagonal so every row index stays in [j, m)
+        if (i0 >= m) i0 = j;
+        const int piv = i0;
         if (t == 0) {
-            R b = sval[0];
-            int i0 = sidx[0];
-            for (int v = 1; v < W; ++v)
-                if (sval[v] > b || (sval[v] == b && sidx[v] < i0)) {
-                    b = sval[v];
-                    i0 = sidx[v];
-                }
-            // a column with no comparable entry (all NaN) yields no candidate:
-            // keep the diagonal so every row index stays in [j, m)
-            if (i0 >= m) i0 = j;
-            spiv = i0;
-            // zero pivot as LAPACK; a NaN pivot is reported the same way
-            if (!(fabs(P[i0 * ld + j]) > (R)0) && info[0] == 0) info[0] = k0 + j + 1;
-            if (i0 != j) {
+            // zero pivot as LAPACK; a NaN pivot (no candidate) is reported the same way
+            if (!(b > (R)0) && info[0] == 0) info[0] = k0 + j + 1;
+            if (piv != j) {
                 const int r = rowid[j];
-                rowid[j] = rowid[i0];
-                rowid[i0] = r;
+                rowid[j] = rowid[piv];
+                rowid[piv] = r;
             }
         }
-        __syncthreads();
-        const int piv = spiv;
         if (piv != j && t < w) {
             const R tmp = P[j * ld + t];
             P[j * ld + t] = P[piv * ld + t];
@@ -382,8 +371,7 @@ __global__ void k_residual(const double *A, const double *x, const double *b, do
     __shared__ double red[4];
     double s = 0.0;
     for (int c = threadIdx.x; c < n; c += blockDim.x) s += A[(size_t)row * n + c] * x[c];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {

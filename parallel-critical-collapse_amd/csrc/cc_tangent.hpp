@@ -260,9 +260,11 @@ template <int N, int T, typename R>
 constexpr int tangent_waves() { return sizeof(R) == 8 ? (N / T >= 8 ? 1 : 2) : (N / T >= 8 ? 2 : 3); }
 
 // SOLO: the launch of the chains that run alone on their CUs (the host pads
-// the launch's LDS); the code is the same, the name tells the launches apart.
+// the launch's LDS): the same code, compiled for one wave per SIMD (no
+// register limit from occupancy, so nothing spills); the name tells the
+// launches apart.
 template <int N, int T, typename R, bool SOLO = false>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(tangent_waves<N, T, R>()))) void k_tangent(TangentParams<R> a)
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(SOLO ? 1 : tangent_waves<N, T, R>()))) void k_tangent(TangentParams<R> a)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
     __shared__ cx<R> xb[(1 + tangent_db<N, R>()) * 2 * N];
