@@ -70,7 +70,14 @@ __device__ __forceinline__ R trcp(R x)
     else return R(1.0) / x;
 }
 
-template <int N, int T, typename R>
+// MERGE (double-buffered only): the two stages' first inverse transforms as
+// one batch of four signals in regions of 4N complex; for workgroups that run
+// alone on their CU (the register and LDS room it needs would cost the
+// chip-filling launch occupancy).
+template <int N, typename R, bool MERGE>
+constexpr int tangent_region() { return (MERGE ? 4 : 2) * N; }
+
+template <int N, int T, typename R, bool MERGE = false>
 __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2], cx<R> (&df)[2][N / T / 2],
                                                  const R *__restrict__ rec, const R (&xs)[2],
                                                  const RhsConst<R> &K, const R *__restrict__ invm,
@@ -82,8 +89,10 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
     asm volatile("" : "+v"(t));
     // exchange regions of the chained transforms (as rhs_pair): no trailing
     // barriers; the caller's reduction barrier ends the evaluation
-    constexpr int DB = tangent_db<N, R>(), RS = 2 * N;
-    constexpr int pA = 0, pB = DB ? lf_next_par<N, T>(pA) : 0, pC = DB ? lf_next_par<N, T>(pB) : 0;
+    constexpr int DB = tangent_db<N, R>(), RS = tangent_region<N, R, MERGE>();
+    static_assert(!MERGE || DB, "the merged transforms need double-buffered regions");
+    constexpr int pA = 0, pB = DB ? lf_next_par<N, T>(pA) : 0;
+    constexpr int pC = DB ? lf_next_par<N, T>(MERGE ? pA : pB) : 0;
     constexpr int pS = DB ? lf_next_par<N, T>(pC) : 0, pG = DB ? 1 - pS : 0;
     constexpr int pH = DB ? lf_next_par<N, T>(pG) : 0;
     const int kt = lf_kt<N, T>(t);      // spectral modes of this thread: kt + T*p
@@ -96,39 +105,9 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
     // the pointwise stage: held across the transforms they would spill)
     R dAU[2][H], dBU[2][H], dAV[2][H], dBV[2][H], dCF[2][H];
     cx<R> c[P];                              // packed h of both stages
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
+    // (3) per stage, after its inverse transform: the linearised split and h_s
+    auto split = [&](int s, const cx<R> (&z)[2][P]) {
         int stage_off = s * RC::STAGE;
-        cx<R> z[2][P];
-        const R fix = bc[s];
-#pragma unroll
-        for (int h = 0; h < H; ++h) {
-            const int q = kt + T * h;
-            const cx<R> Y = dk[s][h];
-            cx<R> lo, hi;
-            if (q < M / 2) {
-                lo = Y;
-                hi = {R(0.0), R(0.0)};
-            } else if (q == M / 2) {
-                lo = cscale(Y, half);
-                hi = Y;
-            } else {
-                lo = {R(0.0), R(0.0)};
-                hi = Y;
-            }
-            if (q == 2) lo.x = -fix;
-            z[0][h] = lo;
-            z[0][h + H] = hi;
-        }
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const int k = kt + T * p;
-            const R w = (k == N / 2) ? R(0.0) : (R)wavenumber<N>(k) * K.k0;
-            const cx<R> v = z[0][p];
-            z[1][p] = {w * v.y, -(w * v.x)};
-        }
-        if (s == 0) lfft_dit<N, T, -1, 2, R, DB, pA, RS, PRE>(z, xb, tw, t, &twr);
-        else lfft_dit<N, T, -1, 2, R, DB, pB, RS, PRE>(z, xb, tw, t, &twr);
         // The record of a step is the same for every fixed-point iteration and
         // every transform; an opaque offset keeps this stage's loads behind its
         // transform instead of hoisted in front of it (or of the loop), where
@@ -168,6 +147,50 @@ __device__ __forceinline__ void tangent_rhs_pair(const cx<R> (&dk)[2][N / T / 2]
             const R hv = -dC1[ph] * rs[RC::E + j] * trcp(rs[RC::R + j]);
             if (s == 0) c[p].x = hv; else c[p].y = hv;
         }
+    };
+    cx<R> zz[2][2][P];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        cx<R>(&z)[2][P] = zz[s];
+        const R fix = bc[s];
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            const int q = kt + T * h;
+            const cx<R> Y = dk[s][h];
+            cx<R> lo, hi;
+            if (q < M / 2) {
+                lo = Y;
+                hi = {R(0.0), R(0.0)};
+            } else if (q == M / 2) {
+                lo = cscale(Y, half);
+                hi = Y;
+            } else {
+                lo = {R(0.0), R(0.0)};
+                hi = Y;
+            }
+            if (q == 2) lo.x = -fix;
+            z[0][h] = lo;
+            z[0][h + H] = hi;
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int k = kt + T * p;
+            const R w = (k == N / 2) ? R(0.0) : (R)wavenumber<N>(k) * K.k0;
+            const cx<R> v = z[0][p];
+            z[1][p] = {w * v.y, -(w * v.x)};
+        }
+        if constexpr (!MERGE) {
+            if (s == 0) lfft_dit<N, T, -1, 2, R, DB, pA, RS, PRE>(z, xb, tw, t, &twr);
+            else lfft_dit<N, T, -1, 2, R, DB, pB, RS, PRE>(z, xb, tw, t, &twr);
+            split(s, z);
+        }
+    }
+    if constexpr (MERGE) {
+        // both stages' inverse transforms as one batch of four signals: one
+        // exchange and barrier instead of two
+        lfft_dit<N, T, -1, 4, R, DB, pA, RS, PRE>(*reinterpret_cast<cx<R>(*)[4][P]>(&zz), xb, tw, t, &twr);
+        split(0, zz[0]);
+        split(1, zz[1]);
     }
     // dIA2 = solveInhom with the base's integrating factor and lambda
     lfft_dif1<N, T, +1, R, DB, pC, RS, PRE>(c, xb, tw, t, &twr);
@@ -260,14 +283,16 @@ template <int N, int T, typename R>
 constexpr int tangent_waves() { return sizeof(R) == 8 ? (N / T >= 8 ? 1 : 2) : (N / T >= 8 ? 2 : 3); }
 
 // SOLO: the launch of the chains that run alone on their CUs (the host pads
-// the launch's LDS): the same code, compiled for one wave per SIMD (no
-// register limit from occupancy, so nothing spills); the name tells the
-// launches apart.
+// the launch's LDS): compiled for one wave per SIMD (no register limit from
+// occupancy, so nothing spills), with both stages' first transforms batched
+// (MERGE); every value is computed as in the main launch.
 template <int N, int T, typename R, bool SOLO = false>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(SOLO ? 1 : tangent_waves<N, T, R>()))) void k_tangent(TangentParams<R> a)
 {
     constexpr int P = N / T, H = P / 2, M = N / 2;
-    __shared__ cx<R> xb[(1 + tangent_db<N, R>()) * 2 * N];
+    // the solo launch batches both stages' first transforms (tangent_rhs_pair)
+    constexpr bool MERGE = SOLO && tangent_db<N, R>() != 0;
+    __shared__ cx<R> xb[(1 + tangent_db<N, R>()) * tangent_region<N, R, MERGE>()];
     __shared__ cx<R> ys[2][M];
     // stage derivatives of the previous step (predictor); at N = 2048 the
     // array would cost a third of the workgroups per CU, so it is left out
@@ -366,7 +391,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(SOLO ? 1 : ta
         int its = 0;
         for (its = 0; its < a.maxIts; ++its) {
             cx<R> df[2][H];
-            tangent_rhs_pair<N, T, R>(dk, df, rec, xs, K, a.invm, xb, bc, a.tw, t);
+            tangent_rhs_pair<N, T, R, MERGE>(dk, df, rec, xs, K, a.invm, xb, bc, a.tw, t);
             R loc = R(0.0), mag = R(0.0);
 #pragma unroll
             for (int h = 0; h < H; ++h) {
