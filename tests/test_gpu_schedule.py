@@ -167,3 +167,31 @@ def test_profile_modes():
         else:
             with pytest.raises(cc.CCError):
                 ctx.base_timing()
+
+
+def test_graph_replay_with_solo_launch():
+    """At Ntau=512 the mixed step splits its slowest tangent chains into a
+    launch of their own on a third stream (from the second Jacobian on): the
+    captured step (three streams, fork and join by events) replays
+    bit-identically to the eager step, three times with eager work between."""
+    s = shape_c2()
+    ctx = ctx_for(s, precision=cc.CC_MIXED)
+    ctx.set_schedule(pipeline=1, steps_per_launch=12, use_graph=0)
+    ctx.profile(True, base=False)
+    ctx.newton_step(s["x"])                   # ranks the chains: the next steps split
+    steps_all = ctx.chain_timing()[3]
+    F0_e, dx_e = ctx.newton_step(s["x"])
+    steps_main = ctx.chain_timing()[3]
+    print("chain steps in the main launches", steps_all, "->", steps_main, flush=True)
+    assert steps_main < steps_all             # the solo launch is active
+    ctx.profile(False)
+    ctx.set_schedule(use_graph=1)
+    x2 = s["x"].copy()
+    x2[3] += 1e-6
+    for rep in range(3):
+        F0_g, dx_g = ctx.newton_step(s["x"])
+        print("replay", rep, flush=True)
+        assert np.array_equal(F0_g, F0_e)
+        assert np.array_equal(dx_g, dx_e)
+        ctx.residual(x2)
+        ctx.synchronize()
