@@ -373,13 +373,14 @@ __device__ __forceinline__ A wave_sum(A v)
 // LAPACK's i?amax).  The rule is a strict order on (value, index), so both
 // lanes of a pair keep the same winner and every lane returns the same pair.
 // Values must not be NaN (callers start from -1 and take only v > best).
+// (Bitwise | and & and selects: branches around the two moves would cost
+// more than the moves.)
 template <typename A>
 __device__ __forceinline__ void argmax_pick(A &v, int &i, A ov, int oi)
 {
-    if (ov > v || (ov == v && oi < i)) {
-        v = ov;
-        i = oi;
-    }
+    const bool take = (ov > v) | ((ov == v) & (oi < i));
+    v = take ? ov : v;
+    i = take ? oi : i;
 }
 template <int CTRL, typename A>
 __device__ __forceinline__ void argmax_dpp(A &v, int &i)

@@ -16,8 +16,9 @@
 //   k_lu_gemm   A22 -= L21 U12, 64x64 tiles per workgroup.  FP64 tiles run on
 //               v_mfma_f64_16x16x4f64, FP32 tiles on v_mfma_f32_16x16x4f32.
 //   k_lu_solve  P b, then blocked forward/back substitution for one
-//               right-hand side (64-row blocks: the off-diagonal part as
-//               wave dot products, the diagonal block by one wave)
+//               right-hand side (right-looking over 128-byte column blocks:
+//               the diagonal block by one wave, the block column's products
+//               by 8-lane groups, one row segment per group and row)
 #pragma once
 #include "cc_device.hpp"
 
@@ -44,6 +45,16 @@ __device__ __forceinline__ double lane_value(double x, int l)
     const int lo = __builtin_amdgcn_readlane((int)v, l);
     const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// v where bit is 1, +0 where it is 0, by an integer mask of v's bits.
+__device__ __forceinline__ float keep_bits(float v, uint32_t bit)
+{
+    return __uint_as_float(__float_as_uint(v) & (0u - bit));
+}
+__device__ __forceinline__ double keep_bits(double v, uint32_t bit)
+{
+    return __longlong_as_double(__double_as_longlong(v) & -(long long)bit);
 }
 
 // Factor columns [k0, k0+w) over rows [k0, n).  perm: row permutation so far
@@ -123,20 +134,35 @@ __global__ __launch_bounds__(LU_PANEL_T) void k_lu_panel(R *A, int n, int k0, in
         }
         __syncthreads();
         // scale column j below the diagonal, then the rank-1 update of the
-        // rest of the row (each row by one thread: no barrier in between)
+        // rest of the row.  Thread t owns rows j+1+t, j+1+t+T, ...: the rows
+        // it scans for the next column's pivot, so that search needs no
+        // barrier first (the next swap comes after the search's barrier).
+        // Each row is read whole in one batch of loads and written whole:
+        // element by element, every update would wait out a load's latency.
+        // The pivot row is zero at columns <= j (and >= w), so those columns
+        // are written back as they were; column j then gets l_ij.  (The LDS
+        // rows hold NB + 1 values; a panel in global memory is always NB wide.)
         const R pv = P[j * ld + j];
         const bool scale = pv != (R)0;
         const R inv = scale ? (R)1 / pv : (R)0;
+        // columns (j, w) as a bit mask; the other columns' values cleared
+        // by integer masks (no branch per column)
+        const uint32_t keep = (w >= 32 ? ~0u : (1u << w) - 1u) & ~((2u << j) - 1u);
+        R pr[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) pr[c] = keep_bits(P[j * ld + c], (keep >> c) & 1u);
         for (int i = j + 1 + t; i < m; i += T) {
+            R a[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) a[c] = P[i * ld + c];
             R lij = P[i * ld + j];
-            if (scale) {
-                lij *= inv;
-                P[i * ld + j] = lij;
-            }
-            for (int c = j + 1; c < w; ++c) P[i * ld + c] -= lij * P[j * ld + c];
+            if (scale) lij *= inv;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) P[i * ld + c] = a[c] - lij * pr[c];
+            P[i * ld + j] = lij;
         }
-        __syncthreads();
     }
+    __syncthreads();
     if constexpr (INLDS) {
         for (int e = t; e < m * w; e += T) {
             const int i = e / w, c = e - i * w;
@@ -295,68 +321,146 @@ __global__ __launch_bounds__(256) void k_lu_gemm(R *A, int n, int k0, int nb)
 }
 
 // Solve with the factors: y = P b (gathered through perm), L y' = y,
-// U x = y'.  One workgroup of 16 waves; b is replaced by x.  info (may be
+// U x = y'.  One workgroup of 1024 threads; b is replaced by x.  info (may be
 // null): when the factorization reported a zero pivot, b is left as it is
 // (dgesv computes no solution then).
+//
+// Right-looking over blocks of SB = 128 bytes of columns (32 FP32 or 16 FP64
+// values), L's blocks top-down, then U's bottom-up.  Per block: every group
+// of 8 lanes loads one 128-byte row segment of the block's column below
+// (above) the diagonal block per row it owns -- loads that do not depend on
+// the solution, the first PF rows' issued before the block's diagonal part is
+// solved -- and wave 0 solves the diagonal block by lane broadcasts; then
+// each group subtracts its rows' products with the block's solution (an
+// 8-lane sum).  The next diagonal block is fetched one block ahead and kept
+// in LDS as the strictly lower (upper) triangle in RB, zero elsewhere, with
+// U's reciprocal diagonal beside it, so the diagonal solve runs without
+// branches: one fused multiply-add per column and lane (U's divisions become
+// multiplications by the reciprocal).
 template <typename R, typename RB>
 __global__ __launch_bounds__(1024) void k_lu_solve(const R *A, int n, const int32_t *perm, RB *b,
                                                    const int32_t *info = nullptr)
 {
+    constexpr int VEC = 16 / (int)sizeof(R);  // values per lane and row segment
+    constexpr int SB = 8 * VEC;               // block width: 8 lanes x 16 bytes
+    constexpr int G = 1024 / 8;               // row groups
+    constexpr int PF = 8;                     // rows per group fetched ahead of the diagonal solve
     __shared__ RB bs[LU_MAXN];
-    __shared__ R D[64][65];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    __shared__ RB D[SB][SB + 1];
+    __shared__ RB rdiag[SB];
+    const int t = threadIdx.x, lane = t & 63, g = t >> 3, q = t & 7;
     if (info && info[0] != 0) return;
+    const int nb = (n + SB - 1) / SB;
+    // row segments 16-byte aligned: vector loads
+    const bool vec = ((size_t)n * sizeof(R)) % 16 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0;
+    // block s < nb: L's rows [s SB, ..); s >= nb: U's block from the bottom
+    auto block = [&](int s, int &i0, int &ib) {
+        if (s < nb) {
+            i0 = s * SB;
+            ib = min(SB, n - i0);
+        } else {
+            const int i1 = n - (s - nb) * SB;
+            i0 = max(0, i1 - SB);
+            ib = i1 - i0;
+        }
+    };
+    // element (t / SB, t % SB) of block s's diagonal part (0 outside it)
+    auto diag_elem = [&](int s) -> R {
+        int i0, ib;
+        block(s, i0, ib);
+        const int i = t / SB, l = t % SB;
+        return (t < SB * SB && i < ib && l < ib) ? A[(size_t)(i0 + i) * n + i0 + l] : (R)0;
+    };
+    auto put_diag = [&](int s, R d) {
+        if (t < SB * SB) {
+            const int i = t / SB, l = t % SB;
+            const bool upper = s >= nb;
+            D[i][l] = (upper ? l > i : l < i) ? (RB)d : (RB)0;
+            if (upper && i == l) rdiag[l] = d != (R)0 ? (RB)1 / (RB)d : (RB)0;
+        }
+    };
     for (int r = t; r < n; r += 1024) bs[r] = b[perm[r]];
+    put_diag(0, diag_elem(0));
+    for (int s = 0; s < 2 * nb; ++s) {
+        const bool upper = s >= nb;
+        int i0, ib;
+        block(s, i0, ib);
+        // rows outside the diagonal block that this block's column updates;
+        // there are none unless the block is full (a partial block is L's
+        // last or U's first)
+        const int rb = upper ? 0 : i0 + ib, re = upper ? i0 : n;
+        const int np = (re - rb + G - 1) / G;
+        // this lane's part of row r's segment in the block column
+        auto segment = [&](int r, R(&a)[VEC]) {
+            const R *src = A + (size_t)r * n + i0 + q * VEC;
+            if (vec) {
+                if constexpr (VEC == 4) {
+                    const float4 v = *reinterpret_cast<const float4 *>(src);
+                    a[0] = v.x, a[1] = v.y, a[2] = v.z, a[3] = v.w;
+                } else {
+                    const double2 v = *reinterpret_cast<const double2 *>(src);
+                    a[0] = v.x, a[1] = v.y;
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) a[v] = src[v];
+            }
+        };
+        R a[PF][VEC];
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+            const int r = rb + g + G * p;
+            if (p < np && r < re) segment(r, a[p]);
+        }
+        const R dnext = s + 1 < 2 * nb ? diag_elem(s + 1) : (R)0;
+        __syncthreads();   // D of this block, bs after the previous block's products
+        if (t < 64) {
+            // lane i < SB holds row i of the block (lanes past ib hold 0 and
+            // meet only zeros of D)
+            const int i = lane & (SB - 1);
+            RB x = lane < ib ? bs[i0 + lane] : (RB)0;
+            if (!upper) {
+#pragma unroll
+                for (int l = 0; l < SB - 1; ++l) x -= D[i][l] * lane_value(x, l);
+            } else {
+                // lane l keeps its value undivided (D's diagonal is 0 here;
+                // no later column touches it) and is divided at the end
+#pragma unroll
+                for (int l = SB - 1; l >= 0; --l) x -= D[i][l] * (lane_value(x, l) * rdiag[l]);
+                x *= rdiag[i];
+            }
+            if (lane < ib) bs[i0 + lane] = x;
+        }
+        __syncthreads();   // the block's solution; every read of D done
+        if (s + 1 < 2 * nb) put_diag(s + 1, dnext);
+        if (np > 0) {
+            RB xv[VEC];
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) xv[v] = bs[i0 + q * VEC + v];
+            auto product = [&](int r, const R(&a)[VEC]) {
+                RB sum = (RB)0;
+                if (r < re) {
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) sum += (RB)a[v] * xv[v];
+                }
+                sum = wave_add_dpp<0xB1>(sum);
+                sum = wave_add_dpp<0x4E>(sum);
+                sum = wave_add_dpp<0x141>(sum);
+                if (q == 0 && r < re) bs[r] -= sum;
+            };
+#pragma unroll
+            for (int p = 0; p < PF; ++p)
+                if (p < np) product(rb + g + G * p, a[p]);
+            // more than PF rows per group (n above PF G + SB): fetched here
+            for (int p = PF; p < np; ++p) {
+                const int r = rb + g + G * p;
+                R e[VEC] = {};
+                if (r < re) segment(r, e);
+                product(r, e);
+            }
+        }
+    }
     __syncthreads();
-    // unit lower L, 64-row blocks top-down
-    for (int i0 = 0; i0 < n; i0 += 64) {
-        const int ib = min(64, n - i0);
-        for (int e = t; e < 64 * 64; e += 1024) {
-            const int i = e >> 6, l = e & 63;
-            D[i][l] = (i < ib && l < ib) ? A[(size_t)(i0 + i) * n + i0 + l] : (R)0;
-        }
-        for (int i = wave; i < ib; i += 16) {
-            RB s = (RB)0;
-            for (int l = lane; l < i0; l += 64) s += (RB)A[(size_t)(i0 + i) * n + l] * bs[l];
-            s = wave_sum(s);
-            if (lane == 0) bs[i0 + i] -= s;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            RB x = lane < ib ? bs[i0 + lane] : (RB)0;
-            for (int l = 0; l < ib - 1; ++l) {
-                const RB xl = lane_value(x, l);
-                if (lane > l) x -= (RB)D[lane][l] * xl;
-            }
-            if (lane < ib) bs[i0 + lane] = x;
-        }
-        __syncthreads();
-    }
-    // upper U, 64-row blocks bottom-up
-    for (int i1 = n; i1 > 0; i1 -= 64) {
-        const int i0 = max(0, i1 - 64), ib = i1 - i0;
-        for (int e = t; e < 64 * 64; e += 1024) {
-            const int i = e >> 6, l = e & 63;
-            D[i][l] = (i < ib && l < ib) ? A[(size_t)(i0 + i) * n + i0 + l] : (R)0;
-        }
-        for (int i = wave; i < ib; i += 16) {
-            RB s = (RB)0;
-            for (int l = i1 + lane; l < n; l += 64) s += (RB)A[(size_t)(i0 + i) * n + l] * bs[l];
-            s = wave_sum(s);
-            if (lane == 0) bs[i0 + i] -= s;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            RB x = lane < ib ? bs[i0 + lane] : (RB)0;
-            for (int l = ib - 1; l >= 0; --l) {
-                if (lane == l) x = x / (RB)D[l][l];
-                const RB xl = lane_value(x, l);
-                if (lane < l) x -= (RB)D[lane][l] * xl;
-            }
-            if (lane < ib) bs[i0 + lane] = x;
-        }
-        __syncthreads();
-    }
     for (int r = t; r < n; r += 1024) b[r] = bs[r];
 }
 
